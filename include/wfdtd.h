@@ -1,0 +1,240 @@
+/*
+ * libwfdtd — C ABI of the MI355X (gfx950) uniform-grid FDTD path.
+ *
+ * Every entry point takes plain device pointers (torch tensors' data_ptr())
+ * and sizes, enqueues on the given HIP stream (void* = hipStream_t; NULL = the
+ * null stream) and returns an int status: WFD_OK (0) or a negative WFD_ERR_*
+ * code, with a description in wfd_last_error().  Nothing throws across the ABI
+ * and no entry point allocates or frees caller buffers.
+ *
+ * Field storage follows the reference: SoA (3, n) float32 with flat index
+ * i = (x*ny + y)*nz + z (z fastest), warp_fdtd/kernels.py:25,100-103.
+ * Citations are to files of the reference (alecjacobson/warp-fdtd).
+ */
+#ifndef WFDTD_H
+#define WFDTD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WFD_OK 0
+#define WFD_ERR_ARGUMENT (-1)    /* bad pointer, size or shape -> ValueError */
+#define WFD_ERR_HIP (-2)         /* HIP runtime error -> RuntimeError */
+#define WFD_ERR_UNSUPPORTED (-3) /* valid but unsupported configuration -> ValueError */
+
+#define WFD_MAX_MONITORS 16
+
+/* ---------------------------------------------------------------- runtime */
+int wfd_version(void);
+/* Thread-local text of the last failure (empty when none). */
+const char* wfd_last_error(void);
+int wfd_device_synchronize(void);
+int wfd_stream_synchronize(void* stream);
+/* Stream capture into a HIP graph (replaces wp.ScopedCapture / wp.capture_launch,
+ * warp_fdtd/solver.py:239-258).  begin/end bracket launches on `stream`. */
+int wfd_stream_create(void** stream);
+int wfd_stream_destroy(void* stream);
+int wfd_graph_begin(void* stream);
+int wfd_graph_end(void* stream, void** graph_exec);
+int wfd_graph_launch(void* graph_exec, void* stream);
+int wfd_graph_destroy(void* graph_exec);
+
+/* ------------------------------------------------ Yee half-steps (kernel ABI)
+ * warp_fdtd/kernels.py:77-148, make_update(electric, packed): one thread per
+ * cell over n = nx*ny*nz.  Arguments mirror the Warp kernel's
+ * (f, g, inv_eps, psi, coeff, source, waves, step, hprev, save_h_from,
+ *  nx, ny, nz, source_x, courant, periodic), with the row strides of the 2-D
+ * arrays made explicit.  coeff is (3, coeff_rows, 3) rows (a, b, 1/kappa) for
+ * the full psi layout (6, psi_stride) and (3, coeff_rows, 4) rows
+ * (a, b, 1/kappa, plane rank) for the packed layout (solver.py:16-29).
+ * source/waves/hprev may be dummies when source_x < 0 / save_h_from > steps. */
+int wfd_update_e(float* f, const float* g, const float* inv_eps, float* psi,
+                 int64_t psi_stride, const float* coeff, int coeff_rows,
+                 const float* source, int64_t source_stride, const float* waves,
+                 int64_t waves_stride, const int* step, float* hprev, int save_h_from,
+                 int nx, int ny, int nz, int source_x, float courant, int periodic,
+                 void* stream);
+int wfd_update_h(float* f, const float* g, const float* inv_eps, float* psi,
+                 int64_t psi_stride, const float* coeff, int coeff_rows,
+                 const float* source, int64_t source_stride, const float* waves,
+                 int64_t waves_stride, const int* step, float* hprev, int save_h_from,
+                 int nx, int ny, int nz, int source_x, float courant, int periodic,
+                 void* stream);
+int wfd_update_e_packed(float* f, const float* g, const float* inv_eps, float* psi,
+                        int64_t psi_stride, const float* coeff, int coeff_rows,
+                        const float* source, int64_t source_stride, const float* waves,
+                        int64_t waves_stride, const int* step, float* hprev,
+                        int save_h_from, int nx, int ny, int nz, int source_x,
+                        float courant, int periodic, void* stream);
+int wfd_update_h_packed(float* f, const float* g, const float* inv_eps, float* psi,
+                        int64_t psi_stride, const float* coeff, int coeff_rows,
+                        const float* source, int64_t source_stride, const float* waves,
+                        int64_t waves_stride, const int* step, float* hprev,
+                        int save_h_from, int nx, int ny, int nz, int source_x,
+                        float courant, int periodic, void* stream);
+/* warp_fdtd/kernels.py:151-153 (delta=+1) and adjoint.py:226-228 (delta=-1). */
+int wfd_advance(int* step, int delta, void* stream);
+
+/* ------------------------------------------------------ fused Yee time step
+ * One full step (E half-step, then H half-step) of warp_fdtd/solver.py:194-234
+ * with CPML, plane-source injection and hprev saving, as two column-marching
+ * kernels.  Replaces the pair of wp.launch(update_e/update_h) calls.  The step
+ * counter protocol needs no separate advance launch: counters[0] holds the
+ * number of completed steps (the reference's `step`), counters[1] the step the
+ * H half-step is applying (read by detectors recorded after the step). */
+typedef struct wfd_yee_plan {
+  float* e;               /* (3, n) */
+  float* h;               /* (3, n) */
+  const float* inv_eps;   /* (n) */
+  float* psi_e;           /* (6, n) full-volume CPML state, touched in slabs only */
+  float* psi_h;           /* (6, n) */
+  const float* coeff_e;   /* (3, coeff_rows, 3) rows (a, b, 1/kappa) */
+  const float* coeff_h;   /* (3, coeff_rows, 3) */
+  const float* src_e;     /* (3, ny*nz) E profile (drives the H half-step) */
+  const float* src_h;     /* (3, ny*nz) H profile (drives the E half-step) */
+  const float* wave_e;    /* (3, steps) */
+  const float* wave_h;    /* (3, steps) */
+  int* counters;          /* int32[2] */
+  float* hprev;           /* (3, n); written inside hbox once t >= save_h_from */
+  int hbox[6];            /* x0, x1, y0, y1, z0, z1 (half-open) */
+  int save_h_from;
+  int coeff_rows;
+  int steps;              /* length of the wave tables */
+  int nx, ny, nz;
+  int source_x;           /* < 0: no source */
+  float courant;
+  int x_chunk;            /* planes per thread column; 0 picks a default */
+} wfd_yee_plan;
+
+int wfd_yee_step(const wfd_yee_plan* plan, void* stream);
+
+/* ------------------------------------------------------------- detectors
+ * warp_fdtd/detectors.py:65-106 record(): one monitor, one launch over its
+ * plane.  weights (steps, 2) float32 complex; phasor (6, count, 2) float32;
+ * flux (max index + 1); area (count); indices/weights of length steps.
+ * Reads t = step[0]; skips when t is outside [0, steps) or indices[t] < 0.
+ * Flux sums are reduced per plane in a fixed order. */
+int wfd_record(const float* e, const float* h, const float* prev, const int* step,
+               const float* weights, const int* indices, float* phasor, float* flux,
+               const float* area, int nx, int ny, int nz, int x0, int y0, int z0, int sx,
+               int sy, int sz, int axis, int exact, int is_phasor, int steps,
+               void* stream);
+
+/* All monitors of a solver in one launch (solver.py:235-236 loop). */
+typedef struct wfd_monitor {
+  const int* indices;
+  const float* weights;
+  float* phasor;
+  float* flux;
+  const float* area;
+  int x0, y0, z0, sx, sy, sz;
+  int axis, exact, is_phasor;
+  int steps; /* length of indices/weights; steps past it record nothing */
+} wfd_monitor;
+
+int wfd_record_monitors(const float* e, const float* h, const float* prev,
+                        const int* step, const wfd_monitor* monitors, int count, int nx,
+                        int ny, int nz, void* stream);
+
+/* Multi-frequency running DFT (no reference counterpart; the single-frequency
+ * case is detectors.py:65-106 with a per-frequency weight table).  stage writes
+ * the colocated E,H of plane (x0..,sx..) for step t = step[0] into slot
+ * t % slots of stage (slots, 6, count); accumulate adds, for the m steps
+ * t = T-m .. T-1 with T = step[0], w[t, f] * stage[t % slots] into the FP64
+ * accumulators phasor (nf, 6, count, 2); weights (steps, nf, 2) float64. */
+int wfd_dft_stage(const float* e, const float* h, const float* prev, const int* step,
+                  float* stage, int slots, int nx, int ny, int nz, int x0, int y0, int z0,
+                  int sx, int sy, int sz, int exact, void* stream);
+int wfd_dft_accumulate(const float* stage, int slots, const double* weights, int steps,
+                       int nf, const int* step, int m, double* phasor, int count,
+                       void* stream);
+
+/* warp_fdtd/volume.py:238-302 record_volume(): trilinear, time-centred
+ * phasors at count physical points (positions (count, 3) in cell units). */
+int wfd_record_volume(const float* e, const float* h, const float* prev,
+                      const float* positions, const float* weights, const int* step,
+                      float* phasor, int count, int nx, int ny, int nz, int first,
+                      void* stream);
+
+/* ---------------------------------------------------------------- adjoint
+ * warp_fdtd/adjoint.py:17-43 record_curl: hist[t % hist_steps, c, j] =
+ * courant * curl_backward(H) in the box (x0,y0,z0,sx,sy,sz), t = step[0].
+ * hist_steps = steps restates the reference's full history. */
+int wfd_record_curl(const float* h, float* hist, int hist_steps, const int* step, int nx,
+                    int ny, int nz, int x0, int y0, int z0, int sx, int sy, int sz,
+                    float courant, void* stream);
+/* adjoint.py:46-88 prepare (transposed CPML); d is (6, n). */
+int wfd_adjoint_prepare(const float* q, const float* eps, float* psi, const float* coeff,
+                        int coeff_rows, float* d, int nx, int ny, int nz, float courant,
+                        int electric, void* stream);
+/* adjoint.py:107-162 gather (transposed differences; grad += hist . q in the
+ * box when from_h).  _f64 accumulates the gradient in double precision. */
+int wfd_adjoint_gather(float* q, const float* d, const float* hist, int hist_steps,
+                       float* grad, const int* step, int nx, int ny, int nz, int x0,
+                       int y0, int z0, int sx, int sy, int sz, int from_h, void* stream);
+int wfd_adjoint_gather_f64(float* q, const float* d, const float* hist, int hist_steps,
+                           double* grad, const int* step, int nx, int ny, int nz, int x0,
+                           int y0, int z0, int sx, int sy, int sz, int from_h,
+                           void* stream);
+/* adjoint.py:181-223 inject (transposed detector colocation, atomic scatter);
+ * source (6, count, 2) float32 complex, weights (steps, 2). */
+int wfd_adjoint_inject(float* qe, float* qh, const float* source, const float* weights,
+                       const int* step, int nx, int ny, int nz, int x0, int y0, int z0,
+                       int sy, int sz, int count, int old_h_only, int steps,
+                       void* stream);
+/* adjoint.py:472-490 write_design: plane (sx, sy) into every z-layer of the box. */
+int wfd_write_design(const float* plane, float* eps, int nx, int ny, int nz, int x0,
+                     int y0, int z0, int sx, int sy, int sz, void* stream);
+
+/* ------------------------------------------------------ DrivenUniform path
+ * warp_fdtd/driven_uniform.py:90-274: staggered arrays (e: sum over c of the
+ * E component grids, h likewise), conductivity sponge, Gaussian current on
+ * E_y, PEC exterior.  layout_xyz selects the 3-D (nx+1, ny+1, nz+1) launch of
+ * magnetic_xyz/electric_xyz; otherwise the flat launch of magnetic/electric. */
+typedef struct wfd_guide {
+  int nx, ny, nz;
+  float spacing, dt;
+  float length, width, height;      /* problem.lengths */
+  float core_width, core_height;
+  float core_epsilon, clad_epsilon;
+  float sponge_width, sponge_rate;
+  float source_x, source_y, source_z;             /* source centre */
+  float source_sigma_x, source_sigma_y, source_sigma_z;
+  int normal_average;
+  int layout_xyz;
+} wfd_guide;
+
+int wfd_driven_magnetic(const wfd_guide* g, const float* e, float* h, void* stream);
+int wfd_driven_electric(const wfd_guide* g, const float* h, float* e, const float* wave,
+                        const int* counter, void* stream);
+
+/* warp_fdtd/guide_observer.py:9-25 gather: out[row] = sum_j w[row,j] * v_j with
+ * v = field (center=0) or 0.5*(field+old) (center=1); indices/weights (rows, 8). */
+int wfd_observer_gather(const int* indices, const float* weights, const float* field,
+                        const float* old, int center, float* out, int rows,
+                        void* stream);
+/* warp_fdtd/guide_validation.py:6-20 count_nonfinite: count += #non-finite. */
+int wfd_count_nonfinite(const float* values, int64_t n, int* count, void* stream);
+/* warp_fdtd/periodic.py:95-100 record_mode: signal[step] = e[0,0]; step += 1. */
+int wfd_record_mode(const float* e, float* signal, int* step, void* stream);
+
+/* --------------------------------------------------------------- material
+ * warp_fdtd/material.py:12-55: 19x19 clamped Gaussian filter (weights (19,19))
+ * then the tanh subpixel projection to inv_eps = 1/(1 + 11.25 value); all
+ * (sx, sy) float32.  pullback is the hand-derived reverse mode of both maps
+ * (material.py:93-102 gets it from wp.Tape): grad_p = S^T P^T grad_inv_eps;
+ * scratch holds 4*sx*sy floats. */
+int wfd_material_forward(const float* p, const float* weights, float* filtered,
+                         float* inv_eps, int sx, int sy, float beta, void* stream);
+int wfd_material_pullback(const float* weights, const float* filtered,
+                          const float* grad_inv_eps, float* grad_p, float* scratch,
+                          int sx, int sy, float beta, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WFDTD_H */

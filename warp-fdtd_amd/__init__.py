@@ -1,0 +1,10 @@
+"""MI355X-native (gfx950) drop-in for warp_fdtd's uniform-grid FDTD path.
+
+Python API of warp_fdtd (solver, detectors, adjoint, material, driven_uniform,
+periodic, volume) over hand-written HIP kernels reached through the C ABI of
+libwfdtd.so (include/wfdtd.h).  Device arrays are torch tensors on ROCm.
+"""
+
+from . import _lib, array, kernels, solver, detectors
+
+__all__ = ["_lib", "array", "kernels", "solver", "detectors"]

@@ -1,0 +1,148 @@
+"""ctypes binding of libwfdtd.so (the HIP C-ABI declared in include/wfdtd.h).
+
+The library is loaded from this package directory; there is no fallback: a
+missing or unloadable library raises ImportError/OSError at first use, so the
+product path never runs anywhere but on the HIP kernels.
+"""
+
+import ctypes
+from pathlib import Path
+
+import torch
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libwfdtd.so"
+
+WFD_OK = 0
+WFD_ERR_ARGUMENT = -1
+WFD_ERR_HIP = -2
+WFD_ERR_UNSUPPORTED = -3
+WFD_MAX_MONITORS = 16
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+L = ctypes.c_int64
+F = ctypes.c_float
+
+
+class YeePlan(ctypes.Structure):
+    """wfd_yee_plan (include/wfdtd.h)."""
+
+    _fields_ = [
+        ("e", P), ("h", P), ("inv_eps", P), ("psi_e", P), ("psi_h", P),
+        ("coeff_e", P), ("coeff_h", P), ("src_e", P), ("src_h", P),
+        ("wave_e", P), ("wave_h", P), ("counters", P), ("hprev", P),
+        ("hbox", I * 6), ("save_h_from", I), ("coeff_rows", I), ("steps", I),
+        ("nx", I), ("ny", I), ("nz", I), ("source_x", I), ("courant", F),
+        ("x_chunk", I),
+    ]
+
+
+class MonitorDesc(ctypes.Structure):
+    """wfd_monitor (include/wfdtd.h)."""
+
+    _fields_ = [
+        ("indices", P), ("weights", P), ("phasor", P), ("flux", P), ("area", P),
+        ("x0", I), ("y0", I), ("z0", I), ("sx", I), ("sy", I), ("sz", I),
+        ("axis", I), ("exact", I), ("is_phasor", I), ("steps", I),
+    ]
+
+
+class GuideDesc(ctypes.Structure):
+    """wfd_guide (include/wfdtd.h)."""
+
+    _fields_ = [
+        ("nx", I), ("ny", I), ("nz", I), ("spacing", F), ("dt", F),
+        ("length", F), ("width", F), ("height", F),
+        ("core_width", F), ("core_height", F),
+        ("core_epsilon", F), ("clad_epsilon", F),
+        ("sponge_width", F), ("sponge_rate", F),
+        ("source_x", F), ("source_y", F), ("source_z", F),
+        ("source_sigma_x", F), ("source_sigma_y", F), ("source_sigma_z", F),
+        ("normal_average", I), ("layout_xyz", I),
+    ]
+
+
+_UPDATE = [P, P, P, P, L, P, I, P, L, P, L, P, P, I, I, I, I, I, F, I, P]
+
+SIGNATURES = {
+    "wfd_version": [],
+    "wfd_last_error": [],
+    "wfd_device_synchronize": [],
+    "wfd_stream_synchronize": [P],
+    "wfd_stream_create": [ctypes.POINTER(P)],
+    "wfd_stream_destroy": [P],
+    "wfd_graph_begin": [P],
+    "wfd_graph_end": [P, ctypes.POINTER(P)],
+    "wfd_graph_launch": [P, P],
+    "wfd_graph_destroy": [P],
+    "wfd_update_e": _UPDATE,
+    "wfd_update_h": _UPDATE,
+    "wfd_update_e_packed": _UPDATE,
+    "wfd_update_h_packed": _UPDATE,
+    "wfd_advance": [P, I, P],
+    "wfd_yee_step": [ctypes.POINTER(YeePlan), P],
+    "wfd_record": [P, P, P, P, P, P, P, P, P] + [I] * 13 + [P],
+    "wfd_record_monitors": [P, P, P, P, ctypes.POINTER(MonitorDesc), I, I, I, I, P],
+    "wfd_dft_stage": [P, P, P, P, P] + [I] * 11 + [P],
+    "wfd_dft_accumulate": [P, I, P, I, I, P, I, P, I, P],
+    "wfd_record_volume": [P, P, P, P, P, P, P, I, I, I, I, I, P],
+    "wfd_record_curl": [P, P, I, P] + [I] * 9 + [F, P],
+    "wfd_adjoint_prepare": [P, P, P, P, I, P, I, I, I, F, I, P],
+    "wfd_adjoint_gather": [P, P, P, I, P, P] + [I] * 10 + [P],
+    "wfd_adjoint_gather_f64": [P, P, P, I, P, P] + [I] * 10 + [P],
+    "wfd_adjoint_inject": [P, P, P, P, P] + [I] * 11 + [P],
+    "wfd_write_design": [P, P] + [I] * 9 + [P],
+    "wfd_driven_magnetic": [ctypes.POINTER(GuideDesc), P, P, P],
+    "wfd_driven_electric": [ctypes.POINTER(GuideDesc), P, P, P, P, P],
+    "wfd_observer_gather": [P, P, P, P, I, P, I, P],
+    "wfd_count_nonfinite": [P, L, P, P],
+    "wfd_record_mode": [P, P, P, P],
+    "wfd_material_forward": [P, P, P, P, I, I, F, P],
+    "wfd_material_pullback": [P, P, P, P, P, I, I, F, P],
+}
+
+_lib = None
+
+
+def lib():
+    """Load libwfdtd.so once; raises if it is absent or a symbol is missing."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc, gfx950)")
+        handle = ctypes.CDLL(str(LIB_PATH))
+        for name, args in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_char_p if name == "wfd_last_error" else ctypes.c_int
+        _lib = handle
+    return _lib
+
+
+def call(name, *args):
+    """Invoke a C entry point and map its status to the reference's exceptions."""
+    status = getattr(lib(), name)(*args)
+    if status != WFD_OK:
+        message = lib().wfd_last_error().decode(errors="replace")
+        if status in (WFD_ERR_ARGUMENT, WFD_ERR_UNSUPPORTED):
+            raise ValueError(f"{name}: {message}")
+        raise RuntimeError(f"{name}: {message}")
+
+
+def stream():
+    """The current torch stream as a hipStream_t handle."""
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(a):
+    """Device pointer of a tensor or Array (None passes as NULL)."""
+    if a is None:
+        return None
+    t = getattr(a, "tensor", a)
+    if not t.is_cuda:
+        raise ValueError("libwfdtd arguments must be device tensors")
+    if not t.is_contiguous():
+        raise ValueError("libwfdtd arguments must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())

@@ -1,0 +1,272 @@
+// Exact discrete adjoint of the Yee + CPML step and detector colocation for
+// gfx950 (warp_fdtd/adjoint.py), with an FP64 gradient accumulator variant.
+#include "common.hpp"
+
+namespace wfd {
+namespace {
+
+constexpr int kBlock = 256;
+
+// adjoint.py:17-43
+__global__ __launch_bounds__(kBlock) void k_record_curl(
+    const float* __restrict__ h, float* __restrict__ hist, int hist_steps,
+    const int* __restrict__ step, int nx, int ny, int nz, int x0, int y0, int z0, int sx,
+    int sy, int sz, float courant) {
+  const int dn = sx * sy * sz;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= dn) return;
+  const int n = nx * ny * nz;
+  const int x = x0 + i / (sy * sz);
+  const int y = y0 + (i / sz) % sy;
+  const int z = z0 + i % sz;
+  const F3 v = sample(h, n, x, y, z, nx, ny, nz, 0);
+  const F3 a = sample(h, n, x - 1, y, z, nx, ny, nz, 0);
+  const F3 b = sample(h, n, x, y - 1, z, nx, ny, nz, 0);
+  const F3 c = sample(h, n, x, y, z - 1, nx, ny, nz, 0);
+  const F3 dx{v.x - a.x, v.y - a.y, v.z - a.z};
+  const F3 dy{v.x - b.x, v.y - b.y, v.z - b.z};
+  const F3 dz{v.x - c.x, v.y - c.y, v.z - c.z};
+  const int64_t base = (int64_t)(step[0] % hist_steps) * 3 * dn + i;
+  hist[base] = courant * (dy.z - dz.y);
+  hist[base + dn] = courant * (dz.x - dx.z);
+  hist[base + 2 * dn] = courant * (dx.y - dy.x);
+}
+
+// adjoint.py:46-88 — transposed CPML; d gets the six partial streams.
+__global__ __launch_bounds__(kBlock) void k_prepare(
+    const float* __restrict__ q, const float* __restrict__ eps, float* __restrict__ psi,
+    const float* __restrict__ coeff, int rows, float* __restrict__ d, int nx, int ny, int nz,
+    float courant, int electric) {
+  const int n = nx * ny * nz;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int x = i / (ny * nz);
+  const int y = (i / nz) % ny;
+  const int z = i % nz;
+  const float factor = electric ? courant * eps[i] : -courant;
+  const float v[3] = {factor * q[i], factor * q[n + i], factor * q[2 * n + i]};
+#pragma unroll
+  for (int axis = 0; axis < 3; ++axis) {
+    const int pos = axis == 0 ? x : (axis == 1 ? y : z);
+    const float* co = coeff + (axis * rows + pos) * 3;
+    const int a = (axis + 1) % 3;
+    const int b = (axis + 2) % 3;
+    const float q1 = -v[a];
+    const float q2 = v[b];
+    float d1 = co[2] * q1;
+    float d2 = co[2] * q2;
+    if (co[0] != 0.f) {
+      const float p1 = psi[(2 * axis) * n + i] + q1;
+      const float p2 = psi[(2 * axis + 1) * n + i] + q2;
+      d1 += co[0] * p1;
+      d2 += co[0] * p2;
+      psi[(2 * axis) * n + i] = co[1] * p1;
+      psi[(2 * axis + 1) * n + i] = co[1] * p2;
+    }
+    d[(2 * axis) * n + i] = d1;
+    d[(2 * axis + 1) * n + i] = d2;
+  }
+}
+
+// adjoint.py:91-104
+__device__ __forceinline__ float read_d(const float* __restrict__ d, int n, int slot, int x,
+                                        int y, int z, int nx, int ny, int nz) {
+  if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) return 0.f;
+  return d[slot * n + (x * ny + y) * nz + z];
+}
+
+// adjoint.py:107-162
+template <typename G>
+__global__ __launch_bounds__(kBlock) void k_gather(
+    float* __restrict__ q, const float* __restrict__ d, const float* __restrict__ hist,
+    int hist_steps, G* __restrict__ grad, const int* __restrict__ step, int nx, int ny,
+    int nz, int x0, int y0, int z0, int sx, int sy, int sz, int from_h) {
+  const int n = nx * ny * nz;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int x = i / (ny * nz);
+  const int y = (i / nz) % ny;
+  const int z = i % nz;
+  const int delta = from_h ? -1 : 1;
+  float v0 = (read_d(d, n, 2, x, y + delta, z, nx, ny, nz) - d[2 * n + i]) +
+             (read_d(d, n, 5, x, y, z + delta, nx, ny, nz) - d[5 * n + i]);
+  float v1 = (read_d(d, n, 1, x + delta, y, z, nx, ny, nz) - d[n + i]) +
+             (read_d(d, n, 4, x, y, z + delta, nx, ny, nz) - d[4 * n + i]);
+  float v2 = (read_d(d, n, 0, x + delta, y, z, nx, ny, nz) - d[i]) +
+             (read_d(d, n, 3, x, y + delta, z, nx, ny, nz) - d[3 * n + i]);
+  if (!from_h) {
+    v0 = -v0;
+    v1 = -v1;
+    v2 = -v2;
+  }
+  const float q0 = q[i] + v0, q1 = q[n + i] + v1, q2 = q[2 * n + i] + v2;
+  q[i] = q0;
+  q[n + i] = q1;
+  q[2 * n + i] = q2;
+  const int t = step[0];
+  if (from_h && t >= 0 && x >= x0 && x < x0 + sx && y >= y0 && y < y0 + sy && z >= z0 &&
+      z < z0 + sz) {
+    const int dn = sx * sy * sz;
+    const int j = ((x - x0) * sy + (y - y0)) * sz + (z - z0);
+    const int64_t base = (int64_t)(t % hist_steps) * 3 * dn + j;
+    const float contrib = hist[base] * q0 + hist[base + dn] * q1 + hist[base + 2 * dn] * q2;
+    grad[j] += (G)contrib;
+  }
+}
+
+// adjoint.py:165-223 — transposed colocation stencils scattered atomically.
+__device__ __forceinline__ void add(float* q, int n, int c, int x, int y, int z, int nx,
+                                    int ny, int nz, float value) {
+  if (x >= 0 && y >= 0 && z >= 0 && x < nx && y < ny && z < nz)
+    atomicAdd(q + c * n + (x * ny + y) * nz + z, value);
+}
+
+__global__ __launch_bounds__(kBlock) void k_inject(
+    float* __restrict__ qe, float* __restrict__ qh, const float* __restrict__ source,
+    const float* __restrict__ weights, const int* __restrict__ step, int nx, int ny, int nz,
+    int x0, int y0, int z0, int sy, int sz, int count, int old_h_only, int steps) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= count) return;
+  const int t = step[0];
+  if (t < 0 || t >= steps) return;
+  const float w0 = weights[2 * t], w1 = weights[2 * t + 1];
+  if (!(fabsf(w0) + fabsf(w1) > 0.f)) return;
+  const int n = nx * ny * nz;
+  const int x = x0 + i / (sy * sz);
+  const int y = y0 + (i / sz) % sy;
+  const int z = z0 + i % sz;
+  float v[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const float2 s = reinterpret_cast<const float2*>(source)[c * count + i];
+    v[c] = s.x * w0 - s.y * w1;
+  }
+  if (!old_h_only) {
+    for (int dz = 0; dz < 2; ++dz) {
+      for (int dx = 0; dx < 2; ++dx) add(qe, n, 0, x - dx, y, z + dz, nx, ny, nz, 0.25f * v[0]);
+      for (int dy = 0; dy < 2; ++dy) add(qe, n, 1, x, y - dy, z + dz, nx, ny, nz, 0.25f * v[1]);
+    }
+    add(qe, n, 2, x, y, z, nx, ny, nz, v[2]);
+  }
+  for (int dy = 0; dy < 2; ++dy) add(qh, n, 0, x, y - dy, z, nx, ny, nz, 0.25f * v[3]);
+  for (int dx = 0; dx < 2; ++dx) add(qh, n, 1, x - dx, y, z, nx, ny, nz, 0.25f * v[4]);
+  for (int dx = 0; dx < 2; ++dx)
+    for (int dy = 0; dy < 2; ++dy)
+      for (int dz = 0; dz < 2; ++dz)
+        add(qh, n, 2, x - dx, y - dy, z + dz, nx, ny, nz, 0.0625f * v[5]);
+}
+
+// adjoint.py:472-490
+__global__ __launch_bounds__(kBlock) void k_write_design(const float* __restrict__ plane,
+                                                         float* __restrict__ eps, int ny,
+                                                         int nz, int x0, int y0, int z0,
+                                                         int sx, int sy, int sz) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= sx * sy * sz) return;
+  const int x = i / (sy * sz);
+  const int y = (i / sz) % sy;
+  const int z = i % sz;
+  eps[((x0 + x) * ny + y0 + y) * nz + z0 + z] = plane[x * sy + y];
+}
+
+int check_box(int nx, int ny, int nz, int x0, int y0, int z0, int sx, int sy, int sz) {
+  WFD_REQUIRE(nx > 0 && ny > 0 && nz > 0 && cells(nx, ny, nz) <= kMaxCells, "bad grid");
+  WFD_REQUIRE(sx >= 0 && sy >= 0 && sz >= 0 && x0 >= 0 && y0 >= 0 && z0 >= 0 &&
+                  x0 + sx <= nx && y0 + sy <= ny && z0 + sz <= nz,
+              "box (%d,%d,%d)+(%d,%d,%d) outside the grid", x0, y0, z0, sx, sy, sz);
+  return WFD_OK;
+}
+
+template <typename G>
+int launch_gather(float* q, const float* d, const float* hist, int hist_steps, G* grad,
+                  const int* step, int nx, int ny, int nz, int x0, int y0, int z0, int sx,
+                  int sy, int sz, int from_h, void* stream) {
+  WFD_REQUIRE(q && d && step, "gather: null pointer");
+  if (int s = check_box(nx, ny, nz, x0, y0, z0, sx, sy, sz)) return s;
+  const bool has_box = sx * sy * sz > 0;
+  WFD_REQUIRE(!(from_h && has_box) || (hist && grad && hist_steps > 0),
+              "gather: history and gradient required");
+  const int n = nx * ny * nz;
+  hipLaunchKernelGGL((k_gather<G>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                     as_stream(stream), q, d, hist, hist_steps > 0 ? hist_steps : 1, grad,
+                     step, nx, ny, nz, x0, y0, z0, sx, sy, sz, from_h);
+  WFD_LAUNCH_CHECK("gather");
+  return WFD_OK;
+}
+
+}  // namespace
+}  // namespace wfd
+
+using namespace wfd;
+
+extern "C" {
+
+int wfd_record_curl(const float* h, float* hist, int hist_steps, const int* step, int nx,
+                    int ny, int nz, int x0, int y0, int z0, int sx, int sy, int sz,
+                    float courant, void* stream) {
+  WFD_REQUIRE(h && hist && step && hist_steps > 0, "record_curl: bad args");
+  if (int s = check_box(nx, ny, nz, x0, y0, z0, sx, sy, sz)) return s;
+  const int dn = sx * sy * sz;
+  if (dn == 0) return WFD_OK;
+  hipLaunchKernelGGL(k_record_curl, dim3((dn + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                     as_stream(stream), h, hist, hist_steps, step, nx, ny, nz, x0, y0, z0, sx,
+                     sy, sz, courant);
+  WFD_LAUNCH_CHECK("record_curl");
+  return WFD_OK;
+}
+
+int wfd_adjoint_prepare(const float* q, const float* eps, float* psi, const float* coeff,
+                        int coeff_rows, float* d, int nx, int ny, int nz, float courant,
+                        int electric, void* stream) {
+  WFD_REQUIRE(q && psi && coeff && d && (!electric || eps), "prepare: null pointer");
+  WFD_REQUIRE(nx > 0 && ny > 0 && nz > 0 && cells(nx, ny, nz) <= kMaxCells, "prepare: bad grid");
+  WFD_REQUIRE(coeff_rows >= nx && coeff_rows >= ny && coeff_rows >= nz,
+              "prepare: coeff_rows < max(shape)");
+  const int n = nx * ny * nz;
+  hipLaunchKernelGGL(k_prepare, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                     as_stream(stream), q, eps, psi, coeff, coeff_rows, d, nx, ny, nz, courant,
+                     electric);
+  WFD_LAUNCH_CHECK("prepare");
+  return WFD_OK;
+}
+
+int wfd_adjoint_gather(float* q, const float* d, const float* hist, int hist_steps,
+                       float* grad, const int* step, int nx, int ny, int nz, int x0, int y0,
+                       int z0, int sx, int sy, int sz, int from_h, void* stream) {
+  return launch_gather<float>(q, d, hist, hist_steps, grad, step, nx, ny, nz, x0, y0, z0, sx,
+                              sy, sz, from_h, stream);
+}
+
+int wfd_adjoint_gather_f64(float* q, const float* d, const float* hist, int hist_steps,
+                           double* grad, const int* step, int nx, int ny, int nz, int x0,
+                           int y0, int z0, int sx, int sy, int sz, int from_h, void* stream) {
+  return launch_gather<double>(q, d, hist, hist_steps, grad, step, nx, ny, nz, x0, y0, z0,
+                               sx, sy, sz, from_h, stream);
+}
+
+int wfd_adjoint_inject(float* qe, float* qh, const float* source, const float* weights,
+                       const int* step, int nx, int ny, int nz, int x0, int y0, int z0,
+                       int sy, int sz, int count, int old_h_only, int steps, void* stream) {
+  WFD_REQUIRE(qe && qh && source && weights && step, "inject: null pointer");
+  WFD_REQUIRE(count > 0 && sy > 0 && sz > 0 && count % (sy * sz) == 0, "inject: bad plane");
+  if (int s = check_box(nx, ny, nz, x0, y0, z0, count / (sy * sz), sy, sz)) return s;
+  hipLaunchKernelGGL(k_inject, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                     as_stream(stream), qe, qh, source, weights, step, nx, ny, nz, x0, y0, z0,
+                     sy, sz, count, old_h_only, steps);
+  WFD_LAUNCH_CHECK("inject");
+  return WFD_OK;
+}
+
+int wfd_write_design(const float* plane, float* eps, int nx, int ny, int nz, int x0, int y0,
+                     int z0, int sx, int sy, int sz, void* stream) {
+  WFD_REQUIRE(plane && eps, "write_design: null pointer");
+  if (int s = check_box(nx, ny, nz, x0, y0, z0, sx, sy, sz)) return s;
+  const int dn = sx * sy * sz;
+  if (dn == 0) return WFD_OK;
+  hipLaunchKernelGGL(k_write_design, dim3((dn + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                     as_stream(stream), plane, eps, ny, nz, x0, y0, z0, sx, sy, sz);
+  WFD_LAUNCH_CHECK("write_design");
+  return WFD_OK;
+}
+
+}  // extern "C"

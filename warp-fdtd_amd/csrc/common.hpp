@@ -1,0 +1,91 @@
+// Shared device helpers and C-ABI error plumbing for libwfdtd (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/wfdtd.h"
+
+namespace wfd {
+
+// Thread-local last-error text returned by wfd_last_error().
+void set_error(const char* fmt, ...);
+
+// Argument-check helpers: every C entry returns an int status and never throws.
+#define WFD_REQUIRE(cond, ...)                  \
+  do {                                          \
+    if (!(cond)) {                              \
+      ::wfd::set_error(__VA_ARGS__);            \
+      return WFD_ERR_ARGUMENT;                  \
+    }                                           \
+  } while (0)
+
+#define WFD_LAUNCH_CHECK(what)                                              \
+  do {                                                                      \
+    hipError_t err_ = hipGetLastError();                                    \
+    if (err_ != hipSuccess) {                                               \
+      ::wfd::set_error("%s: %s", what, hipGetErrorString(err_));            \
+      return WFD_ERR_HIP;                                                   \
+    }                                                                       \
+  } while (0)
+
+#define WFD_HIP(call)                                                       \
+  do {                                                                      \
+    hipError_t err_ = (call);                                               \
+    if (err_ != hipSuccess) {                                               \
+      ::wfd::set_error("%s: %s", #call, hipGetErrorString(err_));           \
+      return WFD_ERR_HIP;                                                   \
+    }                                                                       \
+  } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Largest flat index a 32-bit kernel index may address.
+constexpr int64_t kMaxCells = (int64_t(1) << 31) - 1;
+
+inline int64_t cells(int nx, int ny, int nz) { return (int64_t)nx * ny * nz; }
+
+struct F3 {
+  float x, y, z;
+};
+
+// kernels.py:8-26 — zero-padded (or periodic) SoA fetch at (x*ny+y)*nz+z.
+__device__ __forceinline__ F3 sample(const float* __restrict__ f, int n, int x, int y,
+                                     int z, int nx, int ny, int nz, int periodic) {
+  if (periodic) {
+    x = (x + nx) % nx;
+    y = (y + ny) % ny;
+    z = (z + nz) % nz;
+  }
+  if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) return F3{0.f, 0.f, 0.f};
+  const int i = (x * ny + y) * nz + z;
+  return F3{f[i], f[n + i], f[2 * n + i]};
+}
+
+// Exact integer division o / d for 0 <= o < 2^24 through a float reciprocal
+// plus one correction step (the float quotient is within one of the truth).
+__device__ __forceinline__ int div_small(int o, int d, float inv_d) {
+  int q = __float2int_rz((float)o * inv_d);
+  int r = o - q * d;
+  if (r < 0) q -= 1;
+  else if (r >= d) q += 1;
+  return q;
+}
+
+// One CPML partial (kernels.py:29-47): row c = (a, b, 1/kappa); psi updated
+// in place when a != 0.  Kept as separate multiplies and adds (no FMA) to
+// round exactly as the reference arithmetic.
+__device__ __forceinline__ float cpml_term(float d, float a, float b, float ik,
+                                           float* __restrict__ psi_at) {
+  if (a != 0.f) {
+    const float p = b * (*psi_at) + a * d;
+    *psi_at = p;
+    return ik * d + p;
+  }
+  return ik * d;
+}
+
+}  // namespace wfd

@@ -1,0 +1,136 @@
+"""Colocated field/flux detectors on libwfdtd (drop-in for warp_fdtd/detectors.py).
+
+Time accumulation runs on the device: every attached Monitor of a solver is
+recorded by one kernel launch per step (``record_all``); ``record`` is the
+single-monitor kernel with the reference's argument order.
+"""
+
+import ctypes
+
+import numpy as np
+
+from ._lib import MonitorDesc, WFD_MAX_MONITORS, call, ptr, stream
+from .array import Array
+from .kernels import Kernel
+
+
+def _record(e, h, prev, step, weights, indices, phasor, flux, area, nx, ny, nz, x0, y0,
+            z0, sy, sz, axis, exact, is_phasor, dim):
+    if dim % (sy * sz):
+        raise ValueError("record: launch dim must be a whole number of (sy, sz) planes")
+    steps = min(len(indices), len(weights))
+    call("wfd_record", ptr(e), ptr(h), ptr(prev), ptr(step), ptr(weights), ptr(indices),
+         ptr(phasor), ptr(flux), ptr(area), int(nx), int(ny), int(nz), int(x0), int(y0),
+         int(z0), int(dim // (sy * sz)), int(sy), int(sz), int(axis), int(exact),
+         int(is_phasor), int(steps), stream())
+
+
+# detectors.py:65-106 record; the plane's x extent comes from the launch dim.
+record = Kernel("wfd_record", _record, takes_dim=True)
+
+
+class Monitor:
+    """detectors.py:109-211."""
+
+    def __init__(self, solver, obj):
+        self.obj = obj
+        self.name = obj["name"]
+        self.is_phasor = obj["class"] in ("ModeOverlapDetector", "PhasorDetector")
+        if obj["class"] not in ("ModeOverlapDetector", "PhasorDetector",
+                                "PoyntingFluxDetector"):
+            raise ValueError(f"Unsupported detector {obj['class']}")
+        self.shape = tuple(hi - lo for lo, hi in obj["grid_slice"])
+        self.n = int(np.prod(self.shape))
+        self.exact = self.is_phasor or solver.meta["settings"].get("exact_interpolation", False)
+        indices = np.asarray(solver.data[self.name + "/_time_step_to_arr_idx"], np.int32)
+        self.indices_numpy = indices
+        self.indices = Array.from_numpy(indices, np.int32, solver.device)
+        self.first = int(np.flatnonzero(indices >= 0)[0])
+        weights = np.zeros(solver.steps, np.complex64)
+        if self.is_phasor:
+            src = next(o for o in solver.meta["objects"] if o["class"] == "ModePlaneSource")
+            t = np.arange(solver.steps, dtype=np.float32) * np.float32(solver.meta["dt_s"])
+            phase = np.float32(2 * np.pi * src["frequency_hz"]) * t
+            weights = (np.exp(1j * phase).astype(np.complex64)
+                       * np.float32(2 / obj["window_sum"])
+                       * solver.data[self.name + "/_window_at_time_step_arr"])
+        self.weights = Array.from_numpy(np.column_stack((weights.real, weights.imag)),
+                                        np.float32, solver.device)
+        self.phasor = Array.zeros((6, self.n if self.is_phasor else 1, 2), device=solver.device)
+        self.flux = Array.zeros(int(indices.max()) + 1, device=solver.device)
+        self.axis = obj.get("propagation_axis", 2)
+        widths = []
+        for a, axis in enumerate("xyz"):
+            lo, hi = obj["grid_slice"][a]
+            v = (np.diff(solver.data[axis + "_edges_m"])[lo:hi] if a != self.axis
+                 else np.ones(hi - lo, np.float32))
+            sh = [1, 1, 1]
+            sh[a] = len(v)
+            widths.append(v.reshape(sh))
+        self.area_numpy = np.broadcast_to(widths[0] * widths[1] * widths[2], self.shape).copy()
+        if obj["class"] == "ModeOverlapDetector":
+            self.area_numpy /= self.area_numpy.mean()
+        self.area = Array.from_numpy(self.area_numpy.ravel(), np.float32, solver.device)
+
+    def active_steps(self, steps):
+        return self.indices_numpy[:steps] >= 0
+
+    def reset(self):
+        self.phasor.zero_()
+        self.flux.zero_()
+
+    def descriptor(self):
+        lo = [a for a, b in self.obj["grid_slice"]]
+        d = MonitorDesc()
+        d.indices, d.weights = ptr(self.indices), ptr(self.weights)
+        d.phasor, d.flux, d.area = ptr(self.phasor), ptr(self.flux), ptr(self.area)
+        d.x0, d.y0, d.z0 = lo
+        d.sx, d.sy, d.sz = self.shape
+        d.axis, d.exact, d.is_phasor = int(self.axis), int(self.exact), int(self.is_phasor)
+        d.steps = int(len(self.indices_numpy))
+        return d
+
+    def launch(self, s):
+        record_all(s, [self])
+
+    def phasors(self):
+        raw = self.phasor.numpy()
+        return (raw[..., 0] + 1j * raw[..., 1]).reshape(6, *self.shape)
+
+    def power(self, s):
+        ph = self.phasors()
+        em = s.data[self.name + "/_mode_E"][0]
+        hm = s.data[self.name + "/_mode_H"][0]
+        integrand = (np.cross(em, ph[3:].conj(), axis=0)
+                     + np.cross(ph[:3].conj(), hm, axis=0))[self.axis]
+        alpha = np.sum(integrand * self.area_numpy) / 4
+        return float(abs(alpha) ** 2)
+
+
+def record_all(solver, monitors):
+    """All monitors in as few launches as possible (16 per launch)."""
+    for start in range(0, len(monitors), WFD_MAX_MONITORS):
+        group = monitors[start:start + WFD_MAX_MONITORS]
+        descs = (MonitorDesc * len(group))(*[m.descriptor() for m in group])
+        call("wfd_record_monitors", ptr(solver.E), ptr(solver.H), ptr(solver.hprev),
+             ptr(solver.record_step), descs, len(group), *solver.shape, stream())
+
+
+def attach(solver):
+    """detectors.py:214-220"""
+    for obj in solver.meta["objects"]:
+        if obj["class"].endswith("Detector"):
+            m = Monitor(solver, obj)
+            solver.monitors.append(m)
+            if m.exact:
+                solver.save_h_from = min(solver.save_h_from, m.first)
+
+
+def measurements(solver):
+    """detectors.py:223-232"""
+    powers = {m.name: m.power(solver) for m in solver.monitors
+              if m.obj["class"] == "ModeOverlapDetector"}
+    if powers:
+        powers["modal_transmission"] = powers["mode_out"] / powers["mode_in"]
+        powers["modal_reflection"] = powers["mode_reflected"] / powers["mode_in"]
+    return powers

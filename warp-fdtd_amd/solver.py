@@ -1,0 +1,369 @@
+"""Fixed-input 3-D FDTD solver on the HIP kernels of libwfdtd (gfx950).
+
+Drop-in for warp_fdtd/solver.py: same constructor, validation errors,
+attributes and methods.  NumPy prepares the immutable tables on the host; the
+time loop runs as the fused E+H kernel pair (cpml_storage="full") or the
+per-cell packed-CPML kernels (cpml_storage="packed"), replayed through HIP
+graphs captured by this library.
+"""
+
+from pathlib import Path
+import ctypes
+import hashlib
+import json
+import time
+
+import numpy as np
+import torch
+
+from . import kernels as k
+from ._lib import YeePlan, call, ptr, stream
+from .array import Array
+
+
+def pack_coefficients(coeff, shape):
+    """solver.py:16-29 — rank active planes per axis; pad the six histories to
+    the largest slab.  Column 3 holds the small exact integer plane rank."""
+    packed = np.zeros((*coeff.shape[:2], 4), np.float32)
+    packed[..., :3] = coeff
+    slab_sizes = []
+    for axis, length in enumerate(shape):
+        active = coeff[axis, :length, 0] != 0
+        packed[axis, :length, 3] = np.maximum(np.cumsum(active) - 1, 0)
+        slab_sizes.append(int(active.sum()) * (int(np.prod(shape)) // length))
+    return packed, max(1, *slab_sizes)
+
+
+def owned_array_bytes(solver):
+    """solver.py:32-50 — live device arrays owned by the solver and monitors."""
+    seen = set()
+
+    def size(value):
+        if isinstance(value, Array):
+            key = value.tensor.untyped_storage().data_ptr()
+            if key in seen:
+                return 0
+            seen.add(key)
+            return value.tensor.untyped_storage().nbytes()
+        if isinstance(value, torch.Tensor):
+            key = value.untyped_storage().data_ptr()
+            if key in seen:
+                return 0
+            seen.add(key)
+            return value.untyped_storage().nbytes()
+        if isinstance(value, dict):
+            return sum(size(v) for v in value.values())
+        if isinstance(value, (list, tuple)):
+            return sum(size(v) for v in value)
+        return 0
+
+    return size(vars(solver)) + sum(size(vars(m)) for m in solver.monitors)
+
+
+def load_case(case):
+    """Manifest and inputs of a bundle directory, with the sha256 check."""
+    path = Path(case)
+    meta = json.loads((path / "manifest.json").read_text())
+    raw = path / "inputs.npz"
+    if hashlib.sha256(raw.read_bytes()).hexdigest() != meta["inputs_sha256"]:
+        raise ValueError("Input bundle checksum mismatch")
+    with np.load(raw) as data:
+        return path, meta, {n: data[n] for n in data.files}
+
+
+def cpml_rows(meta, data, shape, field):
+    """solver.py:105-123 — (3, max(shape), 3) rows (a, b, 1/kappa) per axis."""
+    coeff = np.zeros((3, max(shape), 3), np.float32)
+    coeff[:, :, 1:] = 1
+    covered = np.zeros((3, max(shape)), bool)
+    for obj in meta["objects"]:
+        if obj["class"] != "PerfectlyMatchedLayer":
+            continue
+        name = obj["name"]
+        dims = data[f"{name}/pml_a_{field}"].shape
+        axis = int(np.argmax(dims))
+        lo, hi = obj["grid_slice"][axis]
+        if covered[axis, lo:hi].any():
+            raise ValueError("Overlapping CPML slabs unsupported")
+        covered[axis, lo:hi] = True
+        for j, key in enumerate((f"pml_a_{field}", f"pml_b_{field}", f"inv_kappa_{field}")):
+            coeff[axis, lo:hi, j] = data[f"{name}/{key}"].ravel()
+    return coeff
+
+
+def source_waves(meta, data, src, steps, field):
+    """solver.py:162-181 — float32 wave table (3, steps) of one source field,
+    with the reference's separate float32 multiply/divide."""
+    name = src["name"]
+    offsets = data[f"{name}/_time_offset_{field}"].reshape(3, -1)
+    if not np.all(offsets == offsets[:, :1]):
+        raise ValueError("Spatially varying source delays unsupported")
+    half_step = np.float32(0.5 if field == "E" else 0.0)
+    t = (np.arange(steps, dtype=np.float32)[None, :] + half_step + offsets[:, :1]) * np.float32(
+        meta["dt_s"])
+    period = np.float32(1 / src["frequency_hz"])
+    phase = (np.float32(2 * np.pi) * t / period + np.float32(src["phase_shift"])
+             + np.float32(src["profile_phase_shift"]))
+    ramp = np.clip(t / np.float32(src["num_startup_periods"] / src["frequency_hz"]), 0, 1)
+    return (ramp * np.cos(phase) * np.float32(src["static_amplitude_factor"])).astype(np.float32)
+
+
+class Solver:
+    """solver.py:53-264 on libwfdtd.
+
+    Extra keyword ``full_hprev``: hprev (H at the start of the step, saved once
+    t >= save_h_from) is kept for every cell, as in the reference; by default
+    the fused path saves it only inside the footprint the attached monitors
+    read (their planes grown by the colocation stencil), which is all the
+    detectors use.  VolumeMonitor or any other monitor type forces the full
+    volume.
+    """
+
+    def __init__(self, case, device="cuda:0", graph_steps=64, cpml_storage="full",
+                 full_hprev=False, x_chunk=0):
+        if cpml_storage not in ("full", "packed"):
+            raise ValueError("cpml_storage must be full or packed")
+        self.cpml_storage = cpml_storage
+        self.path, self.meta, self.data = load_case(case)
+        d, m = self.data, self.meta
+        eps = d["inv_permittivity"]
+        if (eps.ndim != 4 or eps.shape[0] != 1 or not np.all(np.isfinite(eps))
+                or np.any(eps <= 0)):
+            raise ValueError("Only finite positive isotropic permittivity is supported")
+        if d["inv_permeability"].ndim != 0 or d["inv_permeability"] != 1:
+            raise ValueError("Only relative permeability 1 is supported")
+        if m["settings"].get("precision", "float32") != "float32":
+            raise ValueError("Only float32 field bundles supported")
+        dx = m["settings"]["spacing_nm"] * 1e-9
+        for axis in "xyz":
+            if not np.allclose(np.diff(d[f"{axis}_edges_m"]), dx, rtol=1e-4, atol=0):
+                raise ValueError("Only uniform cubic grids supported")
+        self.shape = tuple(int(v) for v in eps.shape[1:])
+        self.n = int(np.prod(self.shape))
+        self.steps = int(m["time_steps"])
+        if self.steps < 1 or not np.isfinite(m["dt_s"]) or m["dt_s"] <= 0:
+            raise ValueError("Positive time step and trajectory length required")
+        factor = m["settings"].get("courant_factor", 0.99)
+        if not np.isfinite(factor) or not 0 < factor <= 0.99:
+            raise ValueError("Require 0 < exported CFL factor <= 0.99")
+        expected_dt = factor / np.sqrt(3) * dx / 299792458.0
+        if not np.isclose(m["dt_s"], expected_dt, rtol=1e-4, atol=0):
+            raise ValueError("Exported time step disagrees with the declared CFL factor")
+        if graph_steps < 1:
+            raise ValueError("graph_steps must be positive")
+        if self.n >= 2 ** 31:
+            raise ValueError("Grids of 2^31 cells or more are unsupported")
+        if not torch.cuda.is_available():
+            raise RuntimeError("libwfdtd needs a HIP device (torch.cuda is unavailable)")
+        self.device = torch.device(device)
+        self.graph_steps = graph_steps
+        self.courant = factor / np.sqrt(3)
+        self.full_hprev = bool(full_hprev)
+        self.x_chunk = int(x_chunk)
+        self.E = Array.zeros((3, self.n), device=self.device)
+        self.H = Array.zeros((3, self.n), device=self.device)
+        self.hprev = Array.zeros((3, self.n), device=self.device)
+        self.eps = Array.from_numpy(eps.reshape(-1), np.float32, self.device)
+        # counters[0] = completed steps (the reference's `step`); counters[1] = the
+        # step the H half-step is applying, read by monitors after the step.
+        self.counters = Array.zeros(2, np.int32, self.device)
+        self.step = self.counters[0:1]
+        self.coeffs = {}
+        self._coeff_rows = {}
+        for field in "EH":
+            coeff = cpml_rows(m, d, self.shape, field)
+            self._coeff_rows[field] = coeff
+            if cpml_storage == "packed":
+                packed, psi_size = pack_coefficients(coeff, self.shape)
+                self.coeffs[field] = Array.from_numpy(packed, np.float32, self.device)
+            else:
+                psi_size = self.n
+                self.coeffs[field] = Array.from_numpy(coeff, np.float32, self.device)
+            setattr(self, "psi_" + field.lower(), Array.zeros((6, psi_size), device=self.device))
+        sources = [o for o in m["objects"] if o["class"] == "ModePlaneSource"]
+        if len(sources) != 1:
+            raise ValueError("Exactly one exported plane mode source required")
+        src = sources[0]
+        if (src["direction"] != "+" or src["propagation_axis"] != 0
+                or src["temporal_profile"] != "SingleFrequencyProfile"):
+            raise ValueError("Only +x single-frequency source supported")
+        self.source_x = src["grid_slice"][0][0]
+        if src["grid_slice"] != [[self.source_x, self.source_x + 1], [0, self.shape[1]],
+                                 [0, self.shape[2]]]:
+            raise ValueError("Source must span complete yz plane")
+        name = src["name"]
+        if not np.array_equal(d[name + "/_time_step_to_on_idx"], np.arange(self.steps)):
+            raise ValueError("Switched source schedules are not supported")
+        if src["num_startup_periods"] <= 0:
+            raise ValueError("Source must have a positive startup ramp")
+        self.sources, self.waves = {}, {}
+        for field in "EH":
+            profile = d[f"{name}/_{field}"]
+            if np.iscomplexobj(profile):
+                raise ValueError("Complex source profiles unsupported")
+            self.sources[field] = Array.from_numpy(profile.reshape(3, -1), np.float32, self.device)
+            self.waves[field] = Array.from_numpy(
+                source_waves(m, d, src, self.steps, field), np.float32, self.device)
+        self.monitors = []
+        self.save_h_from = self.steps
+        self.graphs = {}
+        self._quiet_graphs = {}
+        self._capture_stream = None
+
+    # ------------------------------------------------------------ stepping
+    @property
+    def record_step(self):
+        """Device counter holding the step just applied, as monitors read it."""
+        return self.counters[1:2] if self.cpml_storage == "full" else self.counters[0:1]
+
+    def hprev_box(self):
+        """Half-open (x0, x1, y0, y1, z0, z1) of cells whose hprev is read."""
+        from .detectors import Monitor
+
+        nx, ny, nz = self.shape
+        if self.full_hprev or any(not isinstance(m, Monitor) for m in self.monitors):
+            return (0, nx, 0, ny, 0, nz)
+        lo = [nx, ny, nz]
+        hi = [0, 0, 0]
+        for m in self.monitors:
+            if not m.exact:
+                continue
+            (x0, x1), (y0, y1), (z0, z1) = m.obj["grid_slice"]
+            for a, (a0, a1) in enumerate(((x0 - 1, x1), (y0 - 1, y1), (z0, z1 + 1))):
+                lo[a] = min(lo[a], max(a0, 0))
+                hi[a] = max(hi[a], min(a1, self.shape[a]))
+        if hi[0] <= lo[0]:
+            return (0, 0, 0, 0, 0, 0)
+        return (lo[0], hi[0], lo[1], hi[1], lo[2], hi[2])
+
+    def _plan(self):
+        p = YeePlan()
+        p.e, p.h = ptr(self.E), ptr(self.H)
+        p.inv_eps = ptr(self.eps)
+        p.psi_e, p.psi_h = ptr(self.psi_e), ptr(self.psi_h)
+        p.coeff_e, p.coeff_h = ptr(self.coeffs["E"]), ptr(self.coeffs["H"])
+        p.src_e, p.src_h = ptr(self.sources["E"]), ptr(self.sources["H"])
+        p.wave_e, p.wave_h = ptr(self.waves["E"]), ptr(self.waves["H"])
+        p.counters = ptr(self.counters)
+        p.hprev = ptr(self.hprev)
+        for a, v in enumerate(self.hprev_box()):
+            p.hbox[a] = int(v)
+        p.save_h_from = int(self.save_h_from)
+        p.coeff_rows = int(self.coeffs["E"].shape[1])
+        p.steps = int(self.steps)
+        p.nx, p.ny, p.nz = self.shape
+        p.source_x = int(self.source_x)
+        p.courant = float(self.courant)
+        p.x_chunk = self.x_chunk
+        return p
+
+    def reset(self):
+        for a in (self.E, self.H, self.hprev, self.psi_e, self.psi_h, self.counters):
+            a.zero_()
+        for monitor in self.monitors:
+            monitor.reset()
+        torch.cuda.synchronize(self.device)
+
+    def _step_fields(self):
+        if self.cpml_storage == "full":
+            k.yee_step(self._plan())
+            return
+        common = [self.hprev, self.save_h_from, *self.shape, self.source_x, self.courant, 0]
+        k.update_e_packed(self.E, self.H, self.eps, self.psi_e, self.coeffs["E"],
+                          self.sources["H"], self.waves["H"], self.step, *common)
+        k.update_h_packed(self.H, self.E, self.eps, self.psi_h, self.coeffs["H"],
+                          self.sources["E"], self.waves["E"], self.step, *common)
+
+    def _record(self):
+        from .detectors import Monitor, record_all
+
+        plain = [m for m in self.monitors if isinstance(m, Monitor)]
+        if plain:
+            record_all(self, plain)
+        for m in self.monitors:
+            if not isinstance(m, Monitor):
+                m.launch(self)
+
+    def _finish_step(self):
+        if self.cpml_storage != "full":
+            k.advance(self.step)
+
+    def launch_step(self, record=True):
+        """solver.py:194-237 — E, H, monitors, time advance."""
+        self._step_fields()
+        if record:
+            self._record()
+        self._finish_step()
+
+    def _capture(self, length, record):
+        if self._capture_stream is None:
+            handle = ctypes.c_void_p()
+            call("wfd_stream_create", ctypes.byref(handle))
+            self._capture_stream = handle
+        s = self._capture_stream
+        torch.cuda.synchronize(self.device)
+        call("wfd_graph_begin", s)
+        ext = torch.cuda.ExternalStream(s.value, device=self.device)
+        try:
+            with torch.cuda.stream(ext):
+                for _ in range(length):
+                    self.launch_step(record=record)
+        finally:
+            graph = ctypes.c_void_p()
+            call("wfd_graph_end", s, ctypes.byref(graph))
+        return graph
+
+    def compile(self):
+        """solver.py:239-250 — warm step, reset, capture graph_steps and tail."""
+        start = time.perf_counter()
+        self.launch_step()
+        torch.cuda.synchronize(self.device)
+        self.reset()
+        self._destroy_graphs()
+        for length in {self.graph_steps, self.steps % self.graph_steps} - {0}:
+            self.graphs[length] = self._capture(length, True)
+            if self.monitors:
+                self._quiet_graphs[length] = self._capture(length, False)
+        self.reset()
+        return time.perf_counter() - start
+
+    def _windows(self):
+        """Boolean per step: does any monitor record at that step?"""
+        active = np.zeros(self.steps, bool)
+        for m in self.monitors:
+            active |= m.active_steps(self.steps)
+        return active
+
+    def run(self):
+        """solver.py:252-258 — replay the captured graphs over all steps."""
+        if not self.graphs:
+            raise RuntimeError("compile() must run before run()")
+        active = self._windows()
+        s = stream()
+        t = 0
+        for length in [self.graph_steps] * (self.steps // self.graph_steps) + (
+                [self.steps % self.graph_steps] if self.steps % self.graph_steps else []):
+            quiet = self._quiet_graphs.get(length)
+            graph = quiet if (quiet is not None and not active[t:t + length].any()) \
+                else self.graphs[length]
+            call("wfd_graph_launch", graph, s)
+            t += length
+        torch.cuda.synchronize(self.device)
+
+    def fields(self):
+        return {name: arr.numpy().reshape(3, *self.shape)
+                for name, arr in (("E", self.E), ("H", self.H))}
+
+    def _destroy_graphs(self):
+        for table in (self.graphs, self._quiet_graphs):
+            for g in table.values():
+                call("wfd_graph_destroy", g)
+            table.clear()
+
+    def __del__(self):
+        try:
+            self._destroy_graphs()
+            if self._capture_stream is not None:
+                call("wfd_stream_destroy", self._capture_stream)
+        except Exception:
+            pass
