@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Headline benchmark of the uniform-grid FDTD path on one MI355X.
+
+Workload: the 160x96x80 straight silicon waveguide at 20 nm (500x220 nm Si
+core in oxide, 12-cell CPML on all faces, TE0 modal source from the
+project's eigenmode solver with a Hann-windowed pulse at 1.55 um, modal +
+flux DFT planes at 8 wavelengths 1.50-1.60 um), 4000 time steps.  A bench
+"step" is one Yee time step (E and H of every cell, CPML, source and the DFT
+detectors); inputs are resident in HBM before the timed region.
+
+    python bench.py [--gpus 1] [--steps 4000] [--warmup 256] [--no-cpu]
+
+Prints one JSON line: Mcell-updates/s, the roofline of the fused Yee step
+kernels (algorithmic bytes B = 52 + 32 f_cpml per cell-update, f_cpml =
+sum over axes of 2L/N, over the measured kernel time, against 8 TB/s) and
+the CPU oracle's rate on a bounded sample of the same grid.
+"""
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+SHAPE = (160, 96, 80)
+PML = 12
+SPACING = 20e-9
+STEPS = 4000
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes_per_cell(shape, pml):
+    f_cpml = sum(2 * pml / n for n in shape)
+    return 52.0 + 32.0 * f_cpml, f_cpml
+
+
+def build_case():
+    import __graft_entry__
+
+    pkg = __graft_entry__.package()
+    path = Path(tempfile.gettempdir()) / "wfdtd_bench_waveguide_160x96x80"
+    if not (path / "manifest.json").exists():
+        pkg.cases.waveguide_case(path, shape=SHAPE, spacing=SPACING, steps=STEPS, pml=PML)
+    return pkg, path
+
+
+def kernel_time_per_step(pkg, s, repeats):
+    """Average duration of the fused Yee step (E + H kernel pair) from HIP
+    events recorded on the launch stream around each launch."""
+    import torch
+
+    plan = s._plan()
+    pairs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+             for _ in range(repeats)]
+    for _ in range(8):
+        pkg.kernels.yee_step(plan)
+    for a, b in pairs:
+        a.record()
+        pkg.kernels.yee_step(plan)
+        b.record()
+    torch.cuda.synchronize()
+    times = np.array([a.elapsed_time(b) for a, b in pairs]) * 1e-3
+    return float(np.median(times)), float(np.mean(times))
+
+
+def cpu_baseline(s, budget_s=12.0):
+    """The CPU oracle (a scalar OpenMP restatement of the reference's
+    update_e/update_h) on the same grid and tables for a bounded number of
+    steps; detectors are plane-sized and left out of the sample."""
+    from oracle import kernels as OK
+
+    nx, ny, nz = s.shape
+    n = s.n
+    E = np.zeros((3, n), np.float32)
+    H = np.zeros((3, n), np.float32)
+    hprev = np.zeros((3, 1), np.float32)
+    eps = s.eps.numpy()
+    psi = [np.zeros((6, n), np.float32) for _ in range(2)]
+    coeff = [np.ascontiguousarray(s._coeff_rows[f], np.float32) for f in "EH"]
+    src = {f: s.sources[f].numpy() for f in "EH"}
+    wav = {f: s.waves[f].numpy() for f in "EH"}
+    step = np.zeros(1, np.int32)
+
+    def one():
+        OK.update(True, E, H, eps, psi[0], coeff[0], src["H"], wav["H"], step, hprev,
+                  s.steps + 1, s.shape, s.source_x, s.courant, 0)
+        OK.update(False, H, E, eps, psi[1], coeff[1], src["E"], wav["E"], step, hprev,
+                  s.steps + 1, s.shape, s.source_x, s.courant, 0)
+        step[0] += 1
+
+    t = time.perf_counter()
+    for _ in range(3):
+        one()
+    probe = (time.perf_counter() - t) / 3
+    count = int(max(5, min(2000, budget_s / max(probe, 1e-6))))
+    t = time.perf_counter()
+    for _ in range(count):
+        one()
+    elapsed = time.perf_counter() - t
+    cores = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+    return {"value": n * count / elapsed / 1e6, "unit": "Mcell-updates/s", "cores": cores,
+            "kind": "port",
+            "sample": f"oracle update_e+update_h, {nx}x{ny}x{nz} grid, steps 3..{3 + count} "
+                      f"({elapsed:.1f} s), float32, OpenMP"}
+
+
+def main():
+    p = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=STEPS)
+    p.add_argument("--warmup", type=int, default=256)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle sample")
+    p.add_argument("--kernel-repeats", type=int, default=200)
+    args = p.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build "
+              "targets a single MI355X; multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+    if args.steps < 1 or args.steps > STEPS or args.warmup < 0:
+        p.error(f"--steps must be in 1..{STEPS}, --warmup >= 0")
+    import torch
+
+    pkg, case = build_case()
+    s = pkg.solver.Solver(case)
+    pkg.detectors.attach(s)
+    s.compile()
+    if args.warmup:
+        s.reset()
+        s.run_steps(args.warmup)
+    s.reset()
+    torch.cuda.synchronize()
+    start_ev = torch.cuda.Event(enable_timing=True)
+    end_ev = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start_ev.record()
+    s.run_steps(args.steps, synchronize=False)
+    end_ev.record()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    gpu_elapsed = start_ev.elapsed_time(end_ev) * 1e-3
+    cells = s.n
+    value = cells * args.steps / elapsed / 1e6
+    finite = bool(np.isfinite(s.E.numpy()).all() and np.isfinite(s.H.numpy()).all())
+
+    B, f_cpml = algorithmic_bytes_per_cell(s.shape, PML)
+    median_s, mean_s = kernel_time_per_step(pkg, s, args.kernel_repeats)
+    achieved = cells * B / mean_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_yee_e + k_yee_h (one fused step)",
+                "bytes_per_cell_update": round(B, 2), "f_cpml": round(f_cpml, 4),
+                "kernel_us_per_step_mean": round(mean_s * 1e6, 3),
+                "kernel_us_per_step_median": round(median_s * 1e6, 3)}
+    line = {
+        "metric": "Mcell-updates/s (fused Yee+CPML step, config-2 waveguide)",
+        "value": round(value, 1),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "straight Si waveguide 160x96x80 @ 20 nm, 12-cell CPML, TE0 "
+                               "Hann pulse at 1.55 um, 8-wavelength modal+flux DFT planes",
+                   "grid": list(s.shape), "cells": cells, "time_steps": args.steps,
+                   "gpu_seconds": round(gpu_elapsed, 6), "finite": finite},
+        "roofline": roofline,
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(s),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

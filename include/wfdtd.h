@@ -143,8 +143,10 @@ int wfd_record_monitors(const float* e, const float* h, const float* prev,
  * case is detectors.py:65-106 with a per-frequency weight table).  stage writes
  * the colocated E,H of plane (x0..,sx..) for step t = step[0] into slot
  * t % slots of stage (slots, 6, count); accumulate adds, for the m steps
- * t = T-m .. T-1 with T = step[0], w[t, f] * stage[t % slots] into the FP64
- * accumulators phasor (nf, 6, count, 2); weights (steps, nf, 2) float64. */
+ * t = last-m+1 .. last with last = step[0] (the latest staged step),
+ * w[t, f] * stage[t % slots] into the FP64 accumulators phasor
+ * (nf, 6, count, 2); weights (steps, nf, 2) float64; steps outside
+ * [0, steps) add nothing. */
 int wfd_dft_stage(const float* e, const float* h, const float* prev, const int* step,
                   float* stage, int slots, int nx, int ny, int nz, int x0, int y0, int z0,
                   int sx, int sy, int sz, int exact, void* stream);

@@ -141,32 +141,35 @@ def write_design(plane, eps, shape, box):
 def driven_magnetic(e, h, n, spacing, dt, lengths, width, rate):
     """driven_uniform.py:90-130"""
     real = e.dtype.type
+    n_arr = np.ascontiguousarray(n, np.int32)  # held until the call returns
+    len_arr = np.ascontiguousarray(lengths, e.dtype)
     _fn("driven_magnetic", e)(
-        _ptr(e), _ptr(h), _ptr(np.asarray(n, np.int32)), float(real(spacing)),
-        float(real(dt)), _ptr(np.asarray(lengths, e.dtype)), float(real(width)),
-        float(real(rate)))
+        _ptr(e), _ptr(h), _ptr(n_arr), float(real(spacing)), float(real(dt)), _ptr(len_arr),
+        float(real(width)), float(real(rate)))
 
 
 def driven_electric(h, e, n, spacing, dt, lengths, core, eps_core, eps_clad, width,
                     rate, src, src_sigma, wave, counter, normal_average):
     """driven_uniform.py:133-189"""
     real = e.dtype.type
+    keep = [np.ascontiguousarray(n, np.int32), np.ascontiguousarray(lengths, e.dtype),
+            np.ascontiguousarray(core, e.dtype), np.ascontiguousarray(src, e.dtype),
+            np.ascontiguousarray(src_sigma, e.dtype)]
     _fn("driven_electric", e)(
-        _ptr(h), _ptr(e), _ptr(np.asarray(n, np.int32)), float(real(spacing)),
-        float(real(dt)), _ptr(np.asarray(lengths, e.dtype)),
-        _ptr(np.asarray(core, e.dtype)), float(real(eps_core)), float(real(eps_clad)),
-        float(real(width)), float(real(rate)), _ptr(np.asarray(src, e.dtype)),
-        _ptr(np.asarray(src_sigma, e.dtype)), _ptr(wave), _ptr(counter),
-        int(normal_average))
+        _ptr(h), _ptr(e), _ptr(keep[0]), float(real(spacing)), float(real(dt)),
+        _ptr(keep[1]), _ptr(keep[2]), float(real(eps_core)), float(real(eps_clad)),
+        float(real(width)), float(real(rate)), _ptr(keep[3]), _ptr(keep[4]), _ptr(wave),
+        _ptr(counter), int(normal_average))
 
 
 def epsilon_normal(p, spacing, lengths, core, eps_core, eps_clad, component,
                    dtype=np.float32):
     """driven_uniform.py:62-87"""
     real = np.dtype(dtype).type
+    keep = [np.ascontiguousarray(p, dtype), np.ascontiguousarray(lengths, dtype),
+            np.ascontiguousarray(core, dtype)]
     return getattr(lib(), "ora_epsilon_normal_" + _suffix(np.zeros(1, dtype)))(
-        _ptr(np.asarray(p, dtype)), float(real(spacing)),
-        _ptr(np.asarray(lengths, dtype)), _ptr(np.asarray(core, dtype)),
+        _ptr(keep[0]), float(real(spacing)), _ptr(keep[1]), _ptr(keep[2]),
         float(real(eps_core)), float(real(eps_clad)), int(component))
 
 

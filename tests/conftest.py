@@ -37,6 +37,7 @@ def gpu(pkg):
 def rel_l2(a, b):
     import numpy as np
 
-    a = np.asarray(a, np.float64)
-    b = np.asarray(b, np.float64)
+    kind = np.complex128 if (np.iscomplexobj(a) or np.iscomplexobj(b)) else np.float64
+    a = np.asarray(a, kind)
+    b = np.asarray(b, kind)
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))

@@ -1,0 +1,325 @@
+"""Exact discrete material adjoint on libwfdtd (drop-in for warp_fdtd/adjoint.py).
+
+The reverse sweep is the exact transpose of the Yee + CPML step and of the
+detector colocation.  Two storage schemes for the forward curl(H) the
+gradient needs inside the design box:
+
+* ``checkpoint_every=None`` — the reference's full history, (steps, 3, box)
+  float32 (adjoint.py:256-260);
+* ``checkpoint_every=K`` — E, H and CPML state saved every K steps during the
+  forward; the reverse restores a checkpoint, recomputes its K steps (bitwise,
+  the forward has no atomics) into a K-step curl buffer and walks them back.
+
+The voxel gradient accumulates in float64 (``grad_dtype``).
+"""
+
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from . import kernels as k
+from ._lib import call, ptr, stream
+from .array import Array
+from .kernels import Kernel
+from .solver import Solver
+
+
+# ------------------------------------------------------------- kernel ABI
+def _record_curl(h, hist, step, nx, ny, nz, x0, y0, z0, sx, sy, sz, courant, dim=None):
+    call("wfd_record_curl", ptr(h), ptr(hist), int(hist.shape[0]), ptr(step), int(nx),
+         int(ny), int(nz), int(x0), int(y0), int(z0), int(sx), int(sy), int(sz),
+         float(courant), stream())
+
+
+def _prepare(q, eps, psi, coeff, d, nx, ny, nz, courant, electric):
+    call("wfd_adjoint_prepare", ptr(q), ptr(eps), ptr(psi), ptr(coeff), int(coeff.shape[1]),
+         ptr(d), int(nx), int(ny), int(nz), float(courant), int(electric), stream())
+
+
+def _gather(q, d, hist, grad, step, nx, ny, nz, x0, y0, z0, sx, sy, sz, from_h):
+    t = getattr(grad, "tensor", grad)
+    name = "wfd_adjoint_gather_f64" if t.dtype == torch.float64 else "wfd_adjoint_gather"
+    call(name, ptr(q), ptr(d), ptr(hist), int(hist.shape[0]), ptr(grad), ptr(step), int(nx),
+         int(ny), int(nz), int(x0), int(y0), int(z0), int(sx), int(sy), int(sz), int(from_h),
+         stream())
+
+
+def _inject(qe, qh, source, weights, step, nx, ny, nz, x0, y0, z0, sy, sz, old_h_only,
+            dim):
+    call("wfd_adjoint_inject", ptr(qe), ptr(qh), ptr(source), ptr(weights), ptr(step),
+         int(nx), int(ny), int(nz), int(x0), int(y0), int(z0), int(sy), int(sz), int(dim),
+         int(old_h_only), int(len(weights)), stream())
+
+
+def _retreat(step):
+    call("wfd_advance", ptr(step), -1, stream())
+
+
+def _write_design(plane, eps, nx, ny, nz, x0, y0, z0, sx, sy, sz):
+    call("wfd_write_design", ptr(plane), ptr(eps), int(nx), int(ny), int(nz), int(x0),
+         int(y0), int(z0), int(sx), int(sy), int(sz), stream())
+
+
+def _n(inputs, first):
+    return int(inputs[first]) * int(inputs[first + 1]) * int(inputs[first + 2])
+
+
+record_curl = Kernel("wfd_record_curl", _record_curl,
+                     dim=lambda i: int(i[9]) * int(i[10]) * int(i[11]))
+prepare = Kernel("wfd_adjoint_prepare", _prepare, dim=lambda i: _n(i, 5))
+gather = Kernel("wfd_adjoint_gather", _gather, dim=lambda i: _n(i, 5))
+inject = Kernel("wfd_adjoint_inject", _inject, takes_dim=True)
+retreat = Kernel("wfd_advance", _retreat, dim=lambda i: 1)
+write_design = Kernel("wfd_write_design", _write_design,
+                      dim=lambda i: int(i[8]) * int(i[9]) * int(i[10]))
+
+
+# ---------------------------------------------------------------- driver
+class Adjoint(Solver):
+    """adjoint.py:231-469 on libwfdtd.
+
+    ``box`` (x0, y0, z0, sx, sy, sz) and the design check default to the
+    reference's 20 nm smooth bend; pass ``box`` explicitly (and
+    ``require_reference_bend=False``) for other bundles.
+    """
+
+    def __init__(self, case, checkpoint_every=None, grad_dtype=np.float64, box=None,
+                 require_reference_bend=True, **solver_kw):
+        super().__init__(case, **solver_kw)
+        if require_reference_bend:
+            if (self.shape != (200, 200, 75) or self.meta["settings"]["spacing_nm"] != 20
+                    or self.meta["settings"]["parameterization"] != "smooth"):
+                raise ValueError("Adjoint driver currently supports the 20 nm smooth bend only")
+            settings = self.meta["settings"]
+            required = {"time_fs": 200, "pml_cells": 10, "domain_um": 4, "voxel_nm": 20,
+                        "scene": "bend"}
+            if any(settings.get(key) != value for key, value in required.items()) or \
+                    settings.get("pad_domain", False):
+                raise ValueError("Adjoint driver requires the unpadded 200 fs reference geometry")
+        self.box = tuple(box) if box is not None else (100, 20, 30, 80, 80, 20)
+        x0, y0, z0, sx, sy, sz = self.box
+        if min(self.box) < 0 or x0 + sx > self.shape[0] or y0 + sy > self.shape[1] or \
+                z0 + sz > self.shape[2]:
+            raise ValueError("Design box outside the grid")
+        self.design_n = sx * sy * sz
+        if checkpoint_every is not None and checkpoint_every < 1:
+            raise ValueError("checkpoint_every must be positive")
+        self.checkpoint_every = checkpoint_every
+        K = checkpoint_every or self.steps
+        self.history = Array.empty((min(K, self.steps), 3, self.design_n), device=self.device)
+        self.qe = Array.zeros((3, self.n), device=self.device)
+        self.qh = Array.zeros((3, self.n), device=self.device)
+        self.psie = Array.zeros(self.psi_e.shape, device=self.device)
+        self.psih = Array.zeros(self.psi_h.shape, device=self.device)
+        self.d = Array.empty((6, self.n), device=self.device)
+        self.grad = Array.zeros(self.design_n, np.dtype(grad_dtype), self.device)
+        self.reverse_step = Array.zeros(1, np.int32, self.device)
+        self.adjoint_sources = []
+        self.reverse_graphs = {}
+        self.checkpoints = None
+        if checkpoint_every is not None:
+            count = (self.steps + K - 1) // K
+            shapes = [(3, self.n), (3, self.n), self.psi_e.shape, self.psi_h.shape]
+            self.checkpoints = [[Array.empty(s, device=self.device) for s in shapes]
+                                for _ in range(count)]
+
+    # --------------------------------------------------------- forward
+    def _state(self):
+        return [self.E, self.H, self.psi_e, self.psi_h]
+
+    def launch_step(self, record=True, curl=True):
+        """adjoint.py:271-327: record_curl reads H before this step's H
+        half-step, i.e. before the fused E+H step as well."""
+        if curl:
+            x0, y0, z0, sx, sy, sz = self.box
+            call("wfd_record_curl", ptr(self.H), ptr(self.history), int(self.history.shape[0]),
+                 ptr(self.step), *self.shape, x0, y0, z0, sx, sy, sz, float(self.courant),
+                 stream())
+        self._step_fields()
+        if record:
+            self._record()
+        self._finish_step()
+
+    def compile(self):
+        start = time.perf_counter()
+        self.launch_step()
+        torch.cuda.synchronize(self.device)
+        self.reset()
+        self._destroy_graphs()
+        full = self.checkpoint_every is None
+        lengths = {self.graph_steps, self.steps % self.graph_steps} - {0}
+        if not full:
+            K = self.checkpoint_every
+            lengths |= {K, self.steps % K} - {0}
+        for length in lengths:
+            # forward with monitors; with checkpoints the curl history is only
+            # recorded during recomputation
+            self.graphs[length] = self._capture_fn(
+                length, lambda: self.launch_step(record=True, curl=full))
+            self._quiet_graphs[length] = self._capture_fn(
+                length, lambda: self.launch_step(record=False, curl=full))
+        self._recompute_graphs = {}
+        if not full:
+            for length in {self.checkpoint_every, self.steps % self.checkpoint_every} - {0}:
+                self._recompute_graphs[length] = self._capture_fn(
+                    length, lambda: self.launch_step(record=False, curl=True))
+        self.reset()
+        return time.perf_counter() - start
+
+    def _replay(self, t0, t1, graphs_by_len, quiet_by_len, active):
+        """Replay graphs covering steps [t0, t1) in chunks of the largest length."""
+        s = stream()
+        t = t0
+        lengths = sorted(graphs_by_len, reverse=True)
+        while t < t1:
+            length = next(L for L in lengths if L <= t1 - t)
+            quiet = quiet_by_len.get(length) if quiet_by_len else None
+            g = quiet if (quiet is not None and not active[t:t + length].any()) \
+                else graphs_by_len[length]
+            call("wfd_graph_launch", g, s)
+            t += length
+
+    def run(self):
+        if self.checkpoint_every is None:
+            return super().run()
+        K = self.checkpoint_every
+        active = self._windows()
+        for c, t0 in enumerate(range(0, self.steps, K)):
+            for dst, src in zip(self.checkpoints[c], self._state()):
+                dst.tensor.copy_(src.tensor)
+            t1 = min(t0 + K, self.steps)
+            graphs = {L: g for L, g in self.graphs.items() if L in (K, self.steps % K)}
+            self._replay(t0, t1, graphs, self._quiet_graphs, active)
+        torch.cuda.synchronize(self.device)
+
+    # --------------------------------------------------------- reverse
+    def prepare_sources(self):
+        """adjoint.py:329-366 — host cotangents, refreshed in place."""
+        previous = self.adjoint_sources
+        monitors = {m.name: m for m in self.monitors}
+        pin = monitors["mode_in"].power(self)
+        pout = monitors["mode_out"].power(self)
+        if not np.isfinite(pin) or pin <= 0 or not np.isfinite(pout):
+            raise ValueError("Finite modal powers and positive incident power required")
+        self.adjoint_sources = []
+        for name, factor in [("mode_in", pout / pin ** 2), ("mode_out", -1 / pin)]:
+            m = monitors[name]
+            ph = m.phasors()
+            em = self.data[name + "/_mode_E"][0]
+            hm = self.data[name + "/_mode_H"][0]
+            a, b = (m.axis + 1) % 3, (m.axis + 2) % 3
+            qe = np.zeros_like(em)
+            qh = np.zeros_like(hm)
+            qe[a] = hm[b].conj() / 4
+            qe[b] = -hm[a].conj() / 4
+            qh[b] = em[a].conj() / 4
+            qh[a] = -em[b].conj() / 4
+            coeff = np.concatenate((qe, qh)) * m.area_numpy[None]
+            beta = np.sum(coeff * ph)
+            source = (2 * factor * beta.conjugate() * coeff).reshape(6, -1).astype(np.complex64)
+            host = np.stack((source.real, source.imag), axis=-1)
+            if previous:
+                old_monitor, v = previous[len(self.adjoint_sources)]
+                if old_monitor.name != m.name:
+                    raise ValueError("Changing detector layout requires new graphs")
+                v.assign(host)
+            else:
+                v = Array.from_numpy(host, np.float32, self.device)
+            self.adjoint_sources.append((m, v))
+
+    def inject(self, old):
+        for m, v in self.adjoint_sources:
+            lo = [a for a, b in m.obj["grid_slice"]]
+            _inject(self.qe, self.qh, v, m.weights, self.reverse_step, *self.shape, *lo,
+                    *m.shape[1:], old, dim=m.n)
+
+    def reverse_one(self):
+        """adjoint.py:388-414"""
+        self.inject(0)
+        _prepare(self.qh, self.eps, self.psih, self.coeffs["H"], self.d, *self.shape,
+                 self.courant, 0)
+        _gather(self.qe, self.d, self.history, self.grad, self.reverse_step, *self.shape,
+                *self.box, 1)
+        _prepare(self.qe, self.eps, self.psie, self.coeffs["E"], self.d, *self.shape,
+                 self.courant, 1)
+        _gather(self.qh, self.d, self.history, self.grad, self.reverse_step, *self.shape,
+                *self.box, 0)
+        self.inject(1)
+        _retreat(self.reverse_step)
+
+    def reset_reverse(self):
+        for a in (self.qe, self.qh, self.psie, self.psih, self.grad):
+            a.zero_()
+        self.reverse_step.assign(np.array([self.steps - 1], np.int32))
+        torch.cuda.synchronize(self.device)
+
+    def compile_reverse(self):
+        if self.cpml_storage != "full":
+            raise ValueError("The adjoint needs the full CPML storage")
+        self.reset_reverse()
+        self.reverse_one()
+        torch.cuda.synchronize(self.device)
+        self.reset_reverse()
+        for length in self._reverse_lengths():
+            self.reverse_graphs[length] = self._capture_fn(length, self.reverse_one)
+        self.reset_reverse()
+
+    def _reverse_lengths(self):
+        if self.checkpoint_every is None:
+            return {self.graph_steps, self.steps % self.graph_steps} - {0}
+        return {self.checkpoint_every, self.steps % self.checkpoint_every} - {0}
+
+    def backward(self):
+        s = stream()
+        if self.checkpoint_every is None:
+            for _ in range(self.steps // self.graph_steps):
+                call("wfd_graph_launch", self.reverse_graphs[self.graph_steps], s)
+            if self.steps % self.graph_steps:
+                call("wfd_graph_launch", self.reverse_graphs[self.steps % self.graph_steps], s)
+            torch.cuda.synchronize(self.device)
+            return
+        K = self.checkpoint_every
+        starts = list(range(0, self.steps, K))
+        for c in reversed(range(len(starts))):
+            t0 = starts[c]
+            length = min(K, self.steps - t0)
+            for dst, src in zip(self._state(), self.checkpoints[c]):
+                dst.tensor.copy_(src.tensor)
+            self.counters.tensor[0] = t0
+            call("wfd_graph_launch", self._recompute_graphs[length], s)
+            call("wfd_graph_launch", self.reverse_graphs[length], s)
+        torch.cuda.synchronize(self.device)
+
+    def set_design(self, parameters):
+        from .material import Material
+
+        material = Material(np.asarray(parameters).reshape(80, 80), self.meta["beta"],
+                            device=self.device)
+        x0, y0, z0, sx, sy, sz = self.box
+        _write_design(material.inv_eps, self.eps, *self.shape, *self.box)
+        return material
+
+    def differentiate(self, parameters):
+        """adjoint.py:453-469 — (-T, dT/dp) for a new design."""
+        from .detectors import measurements
+
+        material = self.set_design(parameters)
+        if not self.graphs:
+            self.compile()
+        self.reset()
+        self.run()
+        self.prepare_sources()
+        if not self.reverse_graphs:
+            self.compile_reverse()
+        self.reset_reverse()
+        self.backward()
+        x0, y0, z0, sx, sy, sz = self.box
+        voxel = self.grad.numpy().reshape(sx, sy, sz).sum(axis=2)
+        gradient = material.pullback(voxel)
+        return -measurements(self)["modal_transmission"], gradient
+
+    def voxel_gradient(self):
+        x0, y0, z0, sx, sy, sz = self.box
+        return self.grad.numpy().reshape(sx, sy, sz).sum(axis=2)

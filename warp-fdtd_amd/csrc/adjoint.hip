@@ -114,46 +114,76 @@ __global__ __launch_bounds__(kBlock) void k_gather(
   }
 }
 
-// adjoint.py:165-223 — transposed colocation stencils scattered atomically.
-__device__ __forceinline__ void add(float* q, int n, int c, int x, int y, int z, int nx,
-                                    int ny, int nz, float value) {
-  if (x >= 0 && y >= 0 && z >= 0 && x < nx && y < ny && z < nz)
-    atomicAdd(q + c * n + (x * ny + y) * nz + z, value);
+// adjoint.py:165-223 — transposed colocation.  The reference scatters each
+// plane cell's six weighted values with atomics; here every target cell of
+// the stencil footprint gathers the same contributions in a fixed order, so
+// the reverse sweep is bit-reproducible (same sums, deterministic rounding).
+struct InjectPlane {
+  const float2* source;
+  float w0, w1;
+  int x0, y0, z0, sx, sy, sz, count;
+};
+
+// value of component c at plane cell (x, y, z); 0 outside the plane
+__device__ __forceinline__ float plane_value(const InjectPlane& p, int c, int x, int y, int z) {
+  if (x < p.x0 || y < p.y0 || z < p.z0 || x >= p.x0 + p.sx || y >= p.y0 + p.sy ||
+      z >= p.z0 + p.sz)
+    return 0.f;
+  const int i = ((x - p.x0) * p.sy + (y - p.y0)) * p.sz + (z - p.z0);
+  const float2 s = p.source[c * p.count + i];
+  return s.x * p.w0 - s.y * p.w1;
 }
 
 __global__ __launch_bounds__(kBlock) void k_inject(
     float* __restrict__ qe, float* __restrict__ qh, const float* __restrict__ source,
     const float* __restrict__ weights, const int* __restrict__ step, int nx, int ny, int nz,
     int x0, int y0, int z0, int sy, int sz, int count, int old_h_only, int steps) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= count) return;
+  // targets: x in [x0-1, x0+sx), y in [y0-1, y0+sy), z in [z0, z0+sz+1)
+  const int sx = count / (sy * sz);
+  const int tx0 = x0 - 1, ty0 = y0 - 1;
+  const int tsx = sx + 1, tsy = sy + 1, tsz = sz + 1;
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= tsx * tsy * tsz) return;
   const int t = step[0];
   if (t < 0 || t >= steps) return;
-  const float w0 = weights[2 * t], w1 = weights[2 * t + 1];
-  if (!(fabsf(w0) + fabsf(w1) > 0.f)) return;
+  InjectPlane p;
+  p.w0 = weights[2 * t];
+  p.w1 = weights[2 * t + 1];
+  if (!(fabsf(p.w0) + fabsf(p.w1) > 0.f)) return;
+  const int x = tx0 + k / (tsy * tsz);
+  const int y = ty0 + (k / tsz) % tsy;
+  const int z = z0 + k % tsz;
+  if (x < 0 || y < 0 || x >= nx || y >= ny || z >= nz) return;
+  p.source = reinterpret_cast<const float2*>(source);
+  p.x0 = x0;
+  p.y0 = y0;
+  p.z0 = z0;
+  p.sx = sx;
+  p.sy = sy;
+  p.sz = sz;
+  p.count = count;
   const int n = nx * ny * nz;
-  const int x = x0 + i / (sy * sz);
-  const int y = y0 + (i / sz) % sy;
-  const int z = z0 + i % sz;
-  float v[6];
-#pragma unroll
-  for (int c = 0; c < 6; ++c) {
-    const float2 s = reinterpret_cast<const float2*>(source)[c * count + i];
-    v[c] = s.x * w0 - s.y * w1;
-  }
+  const int i = (x * ny + y) * nz + z;
+  // scatter target (x-dx, y, z+dz) of E_x <- plane cell (x+dx, y, z-dz), etc.
   if (!old_h_only) {
+    float a0 = 0.f, a1 = 0.f;
     for (int dz = 0; dz < 2; ++dz) {
-      for (int dx = 0; dx < 2; ++dx) add(qe, n, 0, x - dx, y, z + dz, nx, ny, nz, 0.25f * v[0]);
-      for (int dy = 0; dy < 2; ++dy) add(qe, n, 1, x, y - dy, z + dz, nx, ny, nz, 0.25f * v[1]);
+      for (int dx = 0; dx < 2; ++dx) a0 += 0.25f * plane_value(p, 0, x + dx, y, z - dz);
+      for (int dy = 0; dy < 2; ++dy) a1 += 0.25f * plane_value(p, 1, x, y + dy, z - dz);
     }
-    add(qe, n, 2, x, y, z, nx, ny, nz, v[2]);
+    qe[i] += a0;
+    qe[n + i] += a1;
+    qe[2 * n + i] += plane_value(p, 2, x, y, z);
   }
-  for (int dy = 0; dy < 2; ++dy) add(qh, n, 0, x, y - dy, z, nx, ny, nz, 0.25f * v[3]);
-  for (int dx = 0; dx < 2; ++dx) add(qh, n, 1, x - dx, y, z, nx, ny, nz, 0.25f * v[4]);
+  float b0 = 0.f, b1 = 0.f, b2 = 0.f;
+  for (int dy = 0; dy < 2; ++dy) b0 += 0.25f * plane_value(p, 3, x, y + dy, z);
+  for (int dx = 0; dx < 2; ++dx) b1 += 0.25f * plane_value(p, 4, x + dx, y, z);
   for (int dx = 0; dx < 2; ++dx)
     for (int dy = 0; dy < 2; ++dy)
-      for (int dz = 0; dz < 2; ++dz)
-        add(qh, n, 2, x - dx, y - dy, z + dz, nx, ny, nz, 0.0625f * v[5]);
+      for (int dz = 0; dz < 2; ++dz) b2 += 0.0625f * plane_value(p, 5, x + dx, y + dy, z - dz);
+  qh[i] += b0;
+  qh[n + i] += b1;
+  qh[2 * n + i] += b2;
 }
 
 // adjoint.py:472-490
@@ -250,7 +280,8 @@ int wfd_adjoint_inject(float* qe, float* qh, const float* source, const float* w
   WFD_REQUIRE(qe && qh && source && weights && step, "inject: null pointer");
   WFD_REQUIRE(count > 0 && sy > 0 && sz > 0 && count % (sy * sz) == 0, "inject: bad plane");
   if (int s = check_box(nx, ny, nz, x0, y0, z0, count / (sy * sz), sy, sz)) return s;
-  hipLaunchKernelGGL(k_inject, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0,
+  const int targets = (count / (sy * sz) + 1) * (sy + 1) * (sz + 1);
+  hipLaunchKernelGGL(k_inject, dim3((targets + kBlock - 1) / kBlock), dim3(kBlock), 0,
                      as_stream(stream), qe, qh, source, weights, step, nx, ny, nz, x0, y0, z0,
                      sy, sz, count, old_h_only, steps);
   WFD_LAUNCH_CHECK("inject");

@@ -195,7 +195,8 @@ __global__ __launch_bounds__(kBlock) void k_dft_stage(
 
 constexpr int kFreqGroup = 8;
 
-// phasor[f, c, i] += sum_t w[t, f] * stage[t % slots, c, i] for t in [T-m, T).
+// phasor[f, c, i] += sum_t w[t, f] * stage[t % slots, c, i] for the m steps
+// t = last-m+1 .. last (last = step[0], the most recently staged step).
 // One thread per (c, i); frequencies in groups of 8 held in FP64 registers.
 __global__ __launch_bounds__(kBlock) void k_dft_accumulate(
     const float* __restrict__ stage, int slots, const double* __restrict__ weights,
@@ -204,12 +205,12 @@ __global__ __launch_bounds__(kBlock) void k_dft_accumulate(
   const int ci = blockIdx.x * kBlock + threadIdx.x;
   if (ci >= 6 * count) return;
   const int f0 = blockIdx.y * kFreqGroup;
-  const int T = step[0];
+  const int last = step[0];
   double re[kFreqGroup], im[kFreqGroup];
 #pragma unroll
   for (int k = 0; k < kFreqGroup; ++k) re[k] = im[k] = 0.0;
   for (int j = 0; j < m; ++j) {
-    const int t = T - m + j;
+    const int t = last - m + 1 + j;
     if (t < 0 || t >= steps) continue;
     const double v = (double)stage[(size_t)(t % slots) * 6 * count + ci];
     const double* w = weights + 2 * ((size_t)t * nf + f0);

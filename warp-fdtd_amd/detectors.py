@@ -71,9 +71,13 @@ class Monitor:
         if obj["class"] == "ModeOverlapDetector":
             self.area_numpy /= self.area_numpy.mean()
         self.area = Array.from_numpy(self.area_numpy.ravel(), np.float32, solver.device)
+        self.shape_of_grid = solver.shape
 
     def active_steps(self, steps):
         return self.indices_numpy[:steps] >= 0
+
+    def footprint(self):
+        return colocation_footprint(self.obj["grid_slice"], self.shape_of_grid)
 
     def reset(self):
         self.phasor.zero_()
@@ -107,6 +111,120 @@ class Monitor:
         return float(abs(alpha) ** 2)
 
 
+def colocation_footprint(grid_slice, shape):
+    """Cells of hprev the exact colocation of a plane reads: x-1..x, y-1..y,
+    z..z+1 around every plane cell (detectors.py:49-56), clipped to the grid."""
+    (x0, x1), (y0, y1), (z0, z1) = grid_slice
+    return (max(x0 - 1, 0), min(x1, shape[0]), max(y0 - 1, 0), min(y1, shape[1]),
+            max(z0, 0), min(z1 + 1, shape[2]))
+
+
+C0 = 299792458.0
+
+
+class DFTMonitor:
+    """Running DFT of the colocated E and H on a plane at several wavelengths,
+    accumulated in float64 (no reference counterpart; one wavelength with the
+    reference's weights is the PhasorDetector of detectors.py:99-103).
+
+    Every step stages the plane's six colocated components; every graph chunk
+    (or every step when stepping eagerly) folds the staged steps into the
+    accumulators: phasor[f, c, i] += sum_t w[t, f] * field_c(t, i) with
+    w[t, f] = exp(i 2 pi f t dt) dt.
+    """
+
+    is_phasor = True
+
+    def __init__(self, solver, obj):
+        self.obj = obj
+        self.name = obj["name"]
+        self.shape = tuple(hi - lo for lo, hi in obj["grid_slice"])
+        self.lo = tuple(lo for lo, hi in obj["grid_slice"])
+        self.n = int(np.prod(self.shape))
+        self.axis = int(obj.get("propagation_axis", 0))
+        self.exact = bool(obj.get("exact_interpolation", True))
+        self.wavelengths = np.asarray(obj["wavelengths_m"], np.float64)
+        self.frequencies = C0 / self.wavelengths
+        self.nf = len(self.wavelengths)
+        self.steps = solver.steps
+        self.first = 0
+        self.dt = float(solver.meta["dt_s"])
+        t = np.arange(solver.steps, dtype=np.float64) * self.dt
+        w = np.exp(2j * np.pi * self.frequencies[None, :] * t[:, None]) * self.dt
+        self.weights = Array.from_numpy(np.stack((w.real, w.imag), axis=-1), np.float64,
+                                        solver.device)
+        self.slots = int(solver.graph_steps)
+        self.stage = Array.zeros((self.slots, 6, self.n), device=solver.device)
+        self.phasor = Array.zeros((self.nf, 6, self.n, 2), np.float64, solver.device)
+        self.shape_of_grid = solver.shape
+        self.data = solver.data
+        widths = []
+        for a, axis in enumerate("xyz"):
+            lo, hi = obj["grid_slice"][a]
+            v = (np.diff(solver.data[axis + "_edges_m"])[lo:hi].astype(np.float64)
+                 if a != self.axis else np.ones(hi - lo))
+            sh = [1, 1, 1]
+            sh[a] = len(v)
+            widths.append(v.reshape(sh))
+        self.area_numpy = np.broadcast_to(widths[0] * widths[1] * widths[2], self.shape).copy()
+        self._flush_every = 1
+        self._pending = 0
+
+    def active_steps(self, steps):
+        return np.ones(steps, bool)
+
+    def footprint(self):
+        return colocation_footprint(self.obj["grid_slice"], self.shape_of_grid)
+
+    def reset(self):
+        self.phasor.zero_()
+        self._pending = 0
+
+    def begin_chunk(self, length):
+        if length > self.slots:
+            raise ValueError("DFT staging holds at most graph_steps steps")
+        self._flush_every = length
+        self._pending = 0
+
+    def end_chunk(self):
+        self._flush_every = 1
+        self._pending = 0
+
+    def launch(self, s):
+        call("wfd_dft_stage", ptr(s.E), ptr(s.H), ptr(s.hprev), ptr(s.record_step),
+             ptr(self.stage), self.slots, *s.shape, *self.lo, *self.shape, int(self.exact),
+             stream())
+        self._pending += 1
+        if self._pending == self._flush_every:
+            call("wfd_dft_accumulate", ptr(self.stage), self.slots, ptr(self.weights),
+                 self.steps, self.nf, ptr(s.record_step), self._pending, ptr(self.phasor),
+                 self.n, stream())
+            self._pending = 0
+
+    def phasors(self):
+        raw = self.phasor.numpy()
+        return (raw[..., 0] + 1j * raw[..., 1]).reshape(self.nf, 6, *self.shape)
+
+    def flux(self):
+        """0.5 Re sum((E x H*)[axis] * area) per wavelength (reference_report.py:89-90)."""
+        ph = self.phasors()
+        s = np.cross(ph[:, :3], ph[:, 3:].conj(), axis=1)[:, self.axis]
+        return 0.5 * np.real(np.sum(s * self.area_numpy[None], axis=tuple(range(1, s.ndim))))
+
+    def mode_amplitudes(self):
+        """Overlap with the stored mode of every wavelength, the reference's
+        alpha = sum((E_m x H* + E* x H_m)[axis])/4 (detectors.py:203-211)."""
+        ph = self.phasors()
+        em = self.data[self.name + "/_mode_E"]
+        hm = self.data[self.name + "/_mode_H"]
+        integrand = (np.cross(em, ph[:, 3:].conj(), axis=1)
+                     + np.cross(ph[:, :3].conj(), hm, axis=1))[:, self.axis]
+        return np.sum(integrand, axis=tuple(range(1, integrand.ndim))) / 4
+
+    def power(self, s=None):
+        return np.abs(self.mode_amplitudes()) ** 2
+
+
 def record_all(solver, monitors):
     """All monitors in as few launches as possible (16 per launch)."""
     for start in range(0, len(monitors), WFD_MAX_MONITORS):
@@ -120,7 +238,8 @@ def attach(solver):
     """detectors.py:214-220"""
     for obj in solver.meta["objects"]:
         if obj["class"].endswith("Detector"):
-            m = Monitor(solver, obj)
+            cls = DFTMonitor if obj["class"] == "DFTPlaneDetector" else Monitor
+            m = cls(solver, obj)
             solver.monitors.append(m)
             if m.exact:
                 solver.save_h_from = min(solver.save_h_from, m.first)

@@ -91,6 +91,27 @@ def cpml_rows(meta, data, shape, field):
     return coeff
 
 
+PULSE_PROFILES = ("GaussianPulseProfile", "HannPulseProfile")
+
+
+def pulse_waves(meta, src, steps, offsets, half):
+    """Pulse wave tables (no reference counterpart; same time sampling as the
+    single-frequency table): Gaussian exp(-(t-t0)^2/2s^2) cos(w (t-t0) + phi)
+    or Hann sin^2(pi t/T) cos(w t + phi) on 0 <= t <= T, in float64, stored
+    as float32."""
+    t = (np.arange(steps, dtype=np.float64)[None, :] + half + offsets[:, :1]) * meta["dt_s"]
+    w = 2 * np.pi * src["frequency_hz"]
+    phi = src["phase_shift"] + src["profile_phase_shift"]
+    if src["temporal_profile"] == "GaussianPulseProfile":
+        u = t - src["delay_s"]
+        wave = np.exp(-0.5 * (u / src["sigma_s"]) ** 2) * np.cos(w * u + phi)
+    else:
+        T = src["duration_s"]
+        inside = (t >= 0) & (t <= T)
+        wave = np.where(inside, np.sin(np.pi * t / T) ** 2, 0.0) * np.cos(w * t + phi)
+    return (wave * src["static_amplitude_factor"]).astype(np.float32)
+
+
 def source_waves(meta, data, src, steps, field):
     """solver.py:162-181 — float32 wave table (3, steps) of one source field,
     with the reference's separate float32 multiply/divide."""
@@ -98,6 +119,8 @@ def source_waves(meta, data, src, steps, field):
     offsets = data[f"{name}/_time_offset_{field}"].reshape(3, -1)
     if not np.all(offsets == offsets[:, :1]):
         raise ValueError("Spatially varying source delays unsupported")
+    if src["temporal_profile"] in PULSE_PROFILES:
+        return pulse_waves(meta, src, steps, offsets, 0.5 if field == "E" else 0.0)
     half_step = np.float32(0.5 if field == "E" else 0.0)
     t = (np.arange(steps, dtype=np.float32)[None, :] + half_step + offsets[:, :1]) * np.float32(
         meta["dt_s"])
@@ -185,8 +208,8 @@ class Solver:
             raise ValueError("Exactly one exported plane mode source required")
         src = sources[0]
         if (src["direction"] != "+" or src["propagation_axis"] != 0
-                or src["temporal_profile"] != "SingleFrequencyProfile"):
-            raise ValueError("Only +x single-frequency source supported")
+                or src["temporal_profile"] not in ("SingleFrequencyProfile", *PULSE_PROFILES)):
+            raise ValueError("Only +x single-frequency or pulsed source supported")
         self.source_x = src["grid_slice"][0][0]
         if src["grid_slice"] != [[self.source_x, self.source_x + 1], [0, self.shape[1]],
                                  [0, self.shape[2]]]:
@@ -194,7 +217,7 @@ class Solver:
         name = src["name"]
         if not np.array_equal(d[name + "/_time_step_to_on_idx"], np.arange(self.steps)):
             raise ValueError("Switched source schedules are not supported")
-        if src["num_startup_periods"] <= 0:
+        if src["temporal_profile"] == "SingleFrequencyProfile" and src["num_startup_periods"] <= 0:
             raise ValueError("Source must have a positive startup ramp")
         self.sources, self.waves = {}, {}
         for field in "EH":
@@ -217,21 +240,24 @@ class Solver:
         return self.counters[1:2] if self.cpml_storage == "full" else self.counters[0:1]
 
     def hprev_box(self):
-        """Half-open (x0, x1, y0, y1, z0, z1) of cells whose hprev is read."""
-        from .detectors import Monitor
-
+        """Half-open (x0, x1, y0, y1, z0, z1) of cells whose hprev is read: the
+        union of the exact monitors' footprints (a monitor without a
+        ``footprint`` reads the whole volume)."""
         nx, ny, nz = self.shape
-        if self.full_hprev or any(not isinstance(m, Monitor) for m in self.monitors):
-            return (0, nx, 0, ny, 0, nz)
+        full = (0, nx, 0, ny, 0, nz)
+        if self.full_hprev:
+            return full
         lo = [nx, ny, nz]
         hi = [0, 0, 0]
         for m in self.monitors:
-            if not m.exact:
+            if not getattr(m, "exact", True):
                 continue
-            (x0, x1), (y0, y1), (z0, z1) = m.obj["grid_slice"]
-            for a, (a0, a1) in enumerate(((x0 - 1, x1), (y0 - 1, y1), (z0, z1 + 1))):
-                lo[a] = min(lo[a], max(a0, 0))
-                hi[a] = max(hi[a], min(a1, self.shape[a]))
+            box = m.footprint() if hasattr(m, "footprint") else None
+            if box is None:
+                return full
+            for a in range(3):
+                lo[a] = min(lo[a], box[2 * a])
+                hi[a] = max(hi[a], box[2 * a + 1])
         if hi[0] <= lo[0]:
             return (0, 0, 0, 0, 0, 0)
         return (lo[0], hi[0], lo[1], hi[1], lo[2], hi[2])
@@ -295,23 +321,32 @@ class Solver:
             self._record()
         self._finish_step()
 
-    def _capture(self, length, record):
+    def _capture_fn(self, length, fn):
+        """HIP graph of `length` calls of fn, captured on a library stream."""
         if self._capture_stream is None:
             handle = ctypes.c_void_p()
             call("wfd_stream_create", ctypes.byref(handle))
             self._capture_stream = handle
         s = self._capture_stream
         torch.cuda.synchronize(self.device)
+        chunked = [m for m in self.monitors if hasattr(m, "begin_chunk")]
+        for m in chunked:
+            m.begin_chunk(length)
         call("wfd_graph_begin", s)
         ext = torch.cuda.ExternalStream(s.value, device=self.device)
         try:
             with torch.cuda.stream(ext):
                 for _ in range(length):
-                    self.launch_step(record=record)
+                    fn()
         finally:
             graph = ctypes.c_void_p()
             call("wfd_graph_end", s, ctypes.byref(graph))
+            for m in chunked:
+                m.end_chunk()
         return graph
+
+    def _capture(self, length, record):
+        return self._capture_fn(length, lambda: self.launch_step(record=record))
 
     def compile(self):
         """solver.py:239-250 — warm step, reset, capture graph_steps and tail."""
@@ -349,6 +384,23 @@ class Solver:
             call("wfd_graph_launch", graph, s)
             t += length
         torch.cuda.synchronize(self.device)
+
+    def run_steps(self, count, synchronize=True):
+        """Advance exactly `count` steps from the current step with graph
+        replays (graphs of missing lengths are captured on first use)."""
+        if not self.graphs:
+            raise RuntimeError("compile() must run before run_steps()")
+        s = stream()
+        left = int(count)
+        while left > 0:
+            length = min(self.graph_steps, left)
+            if length not in self.graphs:
+                # capture records the launches without running them
+                self.graphs[length] = self._capture(length, True)
+            call("wfd_graph_launch", self.graphs[length], s)
+            left -= length
+        if synchronize:
+            torch.cuda.synchronize(self.device)
 
     def fields(self):
         return {name: arr.numpy().reshape(3, *self.shape)
