@@ -118,6 +118,8 @@ def main():
     p.add_argument("--warmup", type=int, default=256)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle sample")
     p.add_argument("--kernel-repeats", type=int, default=200)
+    p.add_argument("--x-chunk", type=int, default=0,
+                   help="planes per thread column of the fused step (0: autotune)")
     args = p.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build "
@@ -128,7 +130,7 @@ def main():
     import torch
 
     pkg, case = build_case()
-    s = pkg.solver.Solver(case)
+    s = pkg.solver.Solver(case, x_chunk=args.x_chunk)
     pkg.detectors.attach(s)
     s.compile()
     if args.warmup:
@@ -157,7 +159,8 @@ def main():
                 "kernel": "k_yee_e + k_yee_h (one fused step)",
                 "bytes_per_cell_update": round(B, 2), "f_cpml": round(f_cpml, 4),
                 "kernel_us_per_step_mean": round(mean_s * 1e6, 3),
-                "kernel_us_per_step_median": round(median_s * 1e6, 3)}
+                "kernel_us_per_step_median": round(median_s * 1e6, 3),
+                "x_chunk": s.x_chunk}
     line = {
         "metric": "Mcell-updates/s (fused Yee+CPML step, config-2 waveguide)",
         "value": round(value, 1),

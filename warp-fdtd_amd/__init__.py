@@ -7,7 +7,7 @@ libwfdtd.so (include/wfdtd.h).  Device arrays are torch tensors on ROCm.
 
 from . import (_lib, adjoint, array, cases, convergence, detectors, driven_uniform,
                guide_observer, guide_problem, guide_validation, kernels, material, modes,
-               periodic, solver, volume)
+               periodic, solver, volume, autotune)
 
 __all__ = ["_lib", "adjoint", "array", "cases", "convergence", "detectors", "driven_uniform",
            "guide_observer", "guide_problem", "guide_validation", "kernels", "material",

@@ -110,12 +110,13 @@ __global__ void k_advance(int* step, int delta) { step[0] += delta; }
 
 // ------------------------------------------------------------------------
 // Fast step.  A block owns a contiguous 256-cell run of one (y, z) plane
-// (offset o = y*nz + z, coalesced for any nz) and marches it through a chunk
-// of x planes.  The x-neighbour of the difference stencil stays in registers
-// across the march; y/z neighbours come from the block's own or adjacent
-// rows through L1/L2.  Logical block ids are striped over the 8 XCDs so each
-// XCD sweeps a contiguous range of x chunks and keeps its neighbours in its
-// own L2.
+// (offset o = y*nz + z, coalesced for any nz) and a chunk of XC x planes.
+// Every load of the chunk is issued before any arithmetic (fully unrolled),
+// so a wave pays one memory round trip for XC planes instead of XC dependent
+// ones; only the field components the difference stencil uses are loaded,
+// and the x-neighbour of plane p is plane p-1 of the same registers.
+// Logical block ids are striped over the 8 XCDs so each XCD sweeps a
+// contiguous range of x chunks and keeps its neighbours in its own L2.
 struct YeeArgs {
   float* e;
   float* h;
@@ -146,149 +147,267 @@ struct YeeArgs {
   int per;    // logical blocks per XCD stripe
 };
 
-__device__ __forceinline__ void load3(const float* __restrict__ f, int n, int i, float& a,
-                                      float& b, float& c) {
-  a = f[i];
-  b = f[n + i];
-  c = f[2 * n + i];
+struct Cell {
+  int o, y, z, x0, nvalid;
+};
+
+// Decode the XCD-striped block id; false for padding blocks and idle lanes.
+__device__ __forceinline__ bool locate(const YeeArgs& a, int xc, int& logical, Cell& c) {
+  const int bid = blockIdx.x;
+  logical = (bid & 7) * a.per + (bid >> 3);
+  if (logical >= a.total) return false;
+  const int chunk = logical / a.segs;
+  c.o = (logical - chunk * a.segs) * kBlock + threadIdx.x;
+  c.x0 = chunk * xc;
+  c.nvalid = min(xc, a.nx - c.x0);
+  if (c.o >= a.P) return true;
+  c.y = div_small(c.o, a.nz, a.inv_nz);
+  c.z = c.o - c.y * a.nz;
+  return true;
 }
 
+template <int XC>
 __global__ __launch_bounds__(kBlock) void k_yee_e(YeeArgs a) {
-  const int bid = blockIdx.x;
-  const int logical = (bid & 7) * a.per + (bid >> 3);
-  if (logical >= a.total) return;
+  int logical;
+  Cell c;
+  if (!locate(a, XC, logical, c)) return;
   const int t = a.counters[0];
   if (logical == 0 && threadIdx.x == 0) a.counters[1] = t;
-  const int chunk = logical / a.segs;
-  const int o = (logical - chunk * a.segs) * kBlock + threadIdx.x;
-  if (o >= a.P) return;
-  const int nz = a.nz, n = a.n, P = a.P;
-  const int y = div_small(o, nz, a.inv_nz);
-  const int z = o - y * nz;
-  const int x0 = chunk * a.x_chunk;
-  const int x1 = min(a.nx, x0 + a.x_chunk);
+  if (c.o >= a.P) return;
+  const int nz = a.nz, n = a.n, P = a.P, o = c.o, y = c.y, z = c.z, x0 = c.x0;
   const int rows = a.rows;
   const float* __restrict__ ce = a.coeff_e;
-  const float ya = ce[(rows + y) * 3], yb = ce[(rows + y) * 3 + 1], yk = ce[(rows + y) * 3 + 2];
-  const float za = ce[(2 * rows + z) * 3], zb = ce[(2 * rows + z) * 3 + 1],
-              zk = ce[(2 * rows + z) * 3 + 2];
-  const float S = a.courant;
   const float* __restrict__ H = a.h;
   float* __restrict__ E = a.e;
   float* __restrict__ psi = a.psi_e;
-  float mx, my, mz;  // H(x-1) at this (y, z)
-  if (x0 > 0) load3(H, n, (x0 - 1) * P + o, mx, my, mz);
-  else mx = my = mz = 0.f;
-  float wave_y = 0.f, wave_z = 0.f;
-  if (a.source_x >= x0 && a.source_x < x1 && t < a.steps) {
+  const float ya = ce[(rows + y) * 3], yb = ce[(rows + y) * 3 + 1], yk = ce[(rows + y) * 3 + 2];
+  const float za = ce[(2 * rows + z) * 3], zb = ce[(2 * rows + z) * 3 + 1],
+              zk = ce[(2 * rows + z) * 3 + 2];
+  // phase 1: every load of the chunk
+  float my = 0.f, mz = 0.f;  // H_y, H_z of plane x0-1
+  if (x0 > 0) {
+    my = H[n + (x0 - 1) * P + o];
+    mz = H[2 * n + (x0 - 1) * P + o];
+  }
+  float hx[XC], hy[XC], hz[XC], yx[XC], yz[XC], zx[XC], zy[XC], ex[XC], ey[XC], ez[XC],
+      ie[XC], ps[XC][6];
+#pragma unroll
+  for (int p = 0; p < XC; ++p) {
+    if (p < c.nvalid) {
+      const int i = (x0 + p) * P + o;
+      hx[p] = H[i];
+      hy[p] = H[n + i];
+      hz[p] = H[2 * n + i];
+      yx[p] = y > 0 ? H[i - nz] : 0.f;
+      yz[p] = y > 0 ? H[2 * n + i - nz] : 0.f;
+      zx[p] = z > 0 ? H[i - 1] : 0.f;
+      zy[p] = z > 0 ? H[n + i - 1] : 0.f;
+      ex[p] = E[i];
+      ey[p] = E[n + i];
+      ez[p] = E[2 * n + i];
+      ie[p] = a.inv_eps[i];
+      const bool ax = ce[(x0 + p) * 3] != 0.f;
+      ps[p][0] = ax ? psi[i] : 0.f;
+      ps[p][1] = ax ? psi[n + i] : 0.f;
+      ps[p][2] = ya != 0.f ? psi[2 * n + i] : 0.f;
+      ps[p][3] = ya != 0.f ? psi[3 * n + i] : 0.f;
+      ps[p][4] = za != 0.f ? psi[4 * n + i] : 0.f;
+      ps[p][5] = za != 0.f ? psi[5 * n + i] : 0.f;
+    }
+  }
+  const bool src = a.source_x >= x0 && a.source_x < x0 + c.nvalid && t < a.steps;
+  float wave_y = 0.f, wave_z = 0.f, sy = 0.f, sz = 0.f;
+  if (src) {
     wave_y = a.wave_h[2 * a.steps + t];
     wave_z = a.wave_h[a.steps + t];
+    sy = a.src_h[2 * P + o];
+    sz = a.src_h[P + o];
   }
-  for (int x = x0; x < x1; ++x) {
-    const int i = x * P + o;
-    float hx, hy, hz;
-    load3(H, n, i, hx, hy, hz);
-    float yx = 0.f, yy = 0.f, yz = 0.f, zx = 0.f, zy = 0.f, zz = 0.f;
-    if (y > 0) load3(H, n, i - nz, yx, yy, yz);
-    if (z > 0) load3(H, n, i - 1, zx, zy, zz);
-    const float dx1 = hy - my, dx2 = hz - mz;
-    const float dy0 = hx - yx, dy2 = hz - yz;
-    const float dz0 = hx - zx, dz1 = hy - zy;
-    const float xa = ce[x * 3], xb = ce[x * 3 + 1], xk = ce[x * 3 + 2];
-    const float dxz = cpml_term(dx2, xa, xb, xk, psi + i);
-    const float dxy = cpml_term(dx1, xa, xb, xk, psi + n + i);
-    const float dyx = cpml_term(dy0, ya, yb, yk, psi + 2 * n + i);
-    const float dyz = cpml_term(dy2, ya, yb, yk, psi + 3 * n + i);
-    const float dzy = cpml_term(dz1, za, zb, zk, psi + 4 * n + i);
-    const float dzx = cpml_term(dz0, za, zb, zk, psi + 5 * n + i);
-    const float ie = a.inv_eps[i];
-    float ex, ey, ez;
-    load3(E, n, i, ex, ey, ez);
-    ex = ex + (S * (dyz - dzy)) * ie;
-    ey = ey + (S * (dzx - dxz)) * ie;
-    ez = ez + (S * (dxy - dyx)) * ie;
-    if (x == a.source_x) {
-      ey += ((a.src_h[2 * P + o] * wave_y) * S) * ie;
-      ez -= ((a.src_h[P + o] * wave_z) * S) * ie;
+  const float S = a.courant;
+  // phase 2: arithmetic in the reference's order, then the stores
+#pragma unroll
+  for (int p = 0; p < XC; ++p) {
+    if (p < c.nvalid) {
+      const int x = x0 + p;
+      const int i = x * P + o;
+      const float py = p == 0 ? my : hy[p - 1 < 0 ? 0 : p - 1];
+      const float pz = p == 0 ? mz : hz[p - 1 < 0 ? 0 : p - 1];
+      const float dx1 = hy[p] - py, dx2 = hz[p] - pz;
+      const float dy0 = hx[p] - yx[p], dy2 = hz[p] - yz[p];
+      const float dz0 = hx[p] - zx[p], dz1 = hy[p] - zy[p];
+      const float xa = ce[x * 3], xb = ce[x * 3 + 1], xk = ce[x * 3 + 2];
+      float dxz, dxy, dyx, dyz, dzy, dzx;
+      if (xa != 0.f) {
+        const float p0 = xb * ps[p][0] + xa * dx2;
+        const float p1 = xb * ps[p][1] + xa * dx1;
+        psi[i] = p0;
+        psi[n + i] = p1;
+        dxz = xk * dx2 + p0;
+        dxy = xk * dx1 + p1;
+      } else {
+        dxz = xk * dx2;
+        dxy = xk * dx1;
+      }
+      if (ya != 0.f) {
+        const float p2 = yb * ps[p][2] + ya * dy0;
+        const float p3 = yb * ps[p][3] + ya * dy2;
+        psi[2 * n + i] = p2;
+        psi[3 * n + i] = p3;
+        dyx = yk * dy0 + p2;
+        dyz = yk * dy2 + p3;
+      } else {
+        dyx = yk * dy0;
+        dyz = yk * dy2;
+      }
+      if (za != 0.f) {
+        const float p4 = zb * ps[p][4] + za * dz1;
+        const float p5 = zb * ps[p][5] + za * dz0;
+        psi[4 * n + i] = p4;
+        psi[5 * n + i] = p5;
+        dzy = zk * dz1 + p4;
+        dzx = zk * dz0 + p5;
+      } else {
+        dzy = zk * dz1;
+        dzx = zk * dz0;
+      }
+      float vx = ex[p] + (S * (dyz - dzy)) * ie[p];
+      float vy = ey[p] + (S * (dzx - dxz)) * ie[p];
+      float vz = ez[p] + (S * (dxy - dyx)) * ie[p];
+      if (src && x == a.source_x) {
+        vy += ((sy * wave_y) * S) * ie[p];
+        vz -= ((sz * wave_z) * S) * ie[p];
+      }
+      E[i] = vx;
+      E[n + i] = vy;
+      E[2 * n + i] = vz;
     }
-    E[i] = ex;
-    E[n + i] = ey;
-    E[2 * n + i] = ez;
-    mx = hx;
-    my = hy;
-    mz = hz;
   }
 }
 
+template <int XC>
 __global__ __launch_bounds__(kBlock) void k_yee_h(YeeArgs a) {
-  const int bid = blockIdx.x;
-  const int logical = (bid & 7) * a.per + (bid >> 3);
-  if (logical >= a.total) return;
+  int logical;
+  Cell c;
+  if (!locate(a, XC, logical, c)) return;
   const int t = a.counters[1];
   if (logical == 0 && threadIdx.x == 0) a.counters[0] = t + 1;
-  const int chunk = logical / a.segs;
-  const int o = (logical - chunk * a.segs) * kBlock + threadIdx.x;
-  if (o >= a.P) return;
-  const int nz = a.nz, ny = a.ny, nx = a.nx, n = a.n, P = a.P;
-  const int y = div_small(o, nz, a.inv_nz);
-  const int z = o - y * nz;
-  const int x0 = chunk * a.x_chunk;
-  const int x1 = min(nx, x0 + a.x_chunk);
+  if (c.o >= a.P) return;
+  const int nz = a.nz, ny = a.ny, nx = a.nx, n = a.n, P = a.P, o = c.o, y = c.y, z = c.z,
+            x0 = c.x0;
   const int rows = a.rows;
   const float* __restrict__ ch = a.coeff_h;
-  const float ya = ch[(rows + y) * 3], yb = ch[(rows + y) * 3 + 1], yk = ch[(rows + y) * 3 + 2];
-  const float za = ch[(2 * rows + z) * 3], zb = ch[(2 * rows + z) * 3 + 1],
-              zk = ch[(2 * rows + z) * 3 + 2];
-  const float S = a.courant;
   const float* __restrict__ E = a.e;
   float* __restrict__ H = a.h;
   float* __restrict__ psi = a.psi_h;
-  const bool save = t >= a.save_h_from && y >= a.hbox[2] && y < a.hbox[3] &&
-                    z >= a.hbox[4] && z < a.hbox[5];
-  float sx, sy, sz;  // E(x) at this (y, z)
-  load3(E, n, x0 * P + o, sx, sy, sz);
-  float wave_y = 0.f, wave_z = 0.f;
-  if (a.source_x >= x0 && a.source_x < x1 && t < a.steps) {
+  const float ya = ch[(rows + y) * 3], yb = ch[(rows + y) * 3 + 1], yk = ch[(rows + y) * 3 + 2];
+  const float za = ch[(2 * rows + z) * 3], zb = ch[(2 * rows + z) * 3 + 1],
+              zk = ch[(2 * rows + z) * 3 + 2];
+  // phase 1: loads.  E planes x0..x0+XC (the last one is the x+1 neighbour).
+  float ex[XC + 1], ey[XC + 1], ez[XC + 1];
+  float qx[XC], qz[XC], rx[XC], ry[XC], hx[XC], hy[XC], hz[XC], ps[XC][6];
+  {
+    const int i = x0 * P + o;
+    ex[0] = E[i];
+    ey[0] = E[n + i];
+    ez[0] = E[2 * n + i];
+  }
+#pragma unroll
+  for (int p = 0; p < XC; ++p) {
+    ex[p + 1] = ey[p + 1] = ez[p + 1] = 0.f;
+    if (p < c.nvalid) {
+      const int x = x0 + p;
+      const int i = x * P + o;
+      if (x + 1 < nx) {
+        if (p + 1 < XC) ex[p + 1] = E[i + P];
+        ey[p + 1] = E[n + i + P];
+        ez[p + 1] = E[2 * n + i + P];
+      }
+      qx[p] = y + 1 < ny ? E[i + nz] : 0.f;
+      qz[p] = y + 1 < ny ? E[2 * n + i + nz] : 0.f;
+      rx[p] = z + 1 < nz ? E[i + 1] : 0.f;
+      ry[p] = z + 1 < nz ? E[n + i + 1] : 0.f;
+      hx[p] = H[i];
+      hy[p] = H[n + i];
+      hz[p] = H[2 * n + i];
+      const bool ax = ch[x * 3] != 0.f;
+      ps[p][0] = ax ? psi[i] : 0.f;
+      ps[p][1] = ax ? psi[n + i] : 0.f;
+      ps[p][2] = ya != 0.f ? psi[2 * n + i] : 0.f;
+      ps[p][3] = ya != 0.f ? psi[3 * n + i] : 0.f;
+      ps[p][4] = za != 0.f ? psi[4 * n + i] : 0.f;
+      ps[p][5] = za != 0.f ? psi[5 * n + i] : 0.f;
+    }
+  }
+  const bool src = a.source_x >= x0 && a.source_x < x0 + c.nvalid && t < a.steps;
+  float wave_y = 0.f, wave_z = 0.f, sy = 0.f, sz = 0.f;
+  if (src) {
     wave_y = a.wave_e[2 * a.steps + t];
     wave_z = a.wave_e[a.steps + t];
+    sy = a.src_e[2 * P + o];
+    sz = a.src_e[P + o];
   }
-  for (int x = x0; x < x1; ++x) {
-    const int i = x * P + o;
-    float px = 0.f, py = 0.f, pz = 0.f, qx = 0.f, qy = 0.f, qz = 0.f, rx = 0.f, ry = 0.f,
-          rz = 0.f;
-    if (x + 1 < nx) load3(E, n, i + P, px, py, pz);
-    if (y + 1 < ny) load3(E, n, i + nz, qx, qy, qz);
-    if (z + 1 < nz) load3(E, n, i + 1, rx, ry, rz);
-    const float dx1 = py - sy, dx2 = pz - sz;
-    const float dy0 = qx - sx, dy2 = qz - sz;
-    const float dz0 = rx - sx, dz1 = ry - sy;
-    const float xa = ch[x * 3], xb = ch[x * 3 + 1], xk = ch[x * 3 + 2];
-    const float dxz = cpml_term(dx2, xa, xb, xk, psi + i);
-    const float dxy = cpml_term(dx1, xa, xb, xk, psi + n + i);
-    const float dyx = cpml_term(dy0, ya, yb, yk, psi + 2 * n + i);
-    const float dyz = cpml_term(dy2, ya, yb, yk, psi + 3 * n + i);
-    const float dzy = cpml_term(dz1, za, zb, zk, psi + 4 * n + i);
-    const float dzx = cpml_term(dz0, za, zb, zk, psi + 5 * n + i);
-    float hx, hy, hz;
-    load3(H, n, i, hx, hy, hz);
-    if (save && x >= a.hbox[0] && x < a.hbox[1]) {
-      a.hprev[i] = hx;
-      a.hprev[n + i] = hy;
-      a.hprev[2 * n + i] = hz;
+  const bool save = t >= a.save_h_from && y >= a.hbox[2] && y < a.hbox[3] &&
+                    z >= a.hbox[4] && z < a.hbox[5];
+  const float S = a.courant;
+#pragma unroll
+  for (int p = 0; p < XC; ++p) {
+    if (p < c.nvalid) {
+      const int x = x0 + p;
+      const int i = x * P + o;
+      const float dx1 = ey[p + 1] - ey[p], dx2 = ez[p + 1] - ez[p];
+      const float dy0 = qx[p] - ex[p], dy2 = qz[p] - ez[p];
+      const float dz0 = rx[p] - ex[p], dz1 = ry[p] - ey[p];
+      const float xa = ch[x * 3], xb = ch[x * 3 + 1], xk = ch[x * 3 + 2];
+      float dxz, dxy, dyx, dyz, dzy, dzx;
+      if (xa != 0.f) {
+        const float p0 = xb * ps[p][0] + xa * dx2;
+        const float p1 = xb * ps[p][1] + xa * dx1;
+        psi[i] = p0;
+        psi[n + i] = p1;
+        dxz = xk * dx2 + p0;
+        dxy = xk * dx1 + p1;
+      } else {
+        dxz = xk * dx2;
+        dxy = xk * dx1;
+      }
+      if (ya != 0.f) {
+        const float p2 = yb * ps[p][2] + ya * dy0;
+        const float p3 = yb * ps[p][3] + ya * dy2;
+        psi[2 * n + i] = p2;
+        psi[3 * n + i] = p3;
+        dyx = yk * dy0 + p2;
+        dyz = yk * dy2 + p3;
+      } else {
+        dyx = yk * dy0;
+        dyz = yk * dy2;
+      }
+      if (za != 0.f) {
+        const float p4 = zb * ps[p][4] + za * dz1;
+        const float p5 = zb * ps[p][5] + za * dz0;
+        psi[4 * n + i] = p4;
+        psi[5 * n + i] = p5;
+        dzy = zk * dz1 + p4;
+        dzx = zk * dz0 + p5;
+      } else {
+        dzy = zk * dz1;
+        dzx = zk * dz0;
+      }
+      if (save && x >= a.hbox[0] && x < a.hbox[1]) {
+        a.hprev[i] = hx[p];
+        a.hprev[n + i] = hy[p];
+        a.hprev[2 * n + i] = hz[p];
+      }
+      float vx = hx[p] - S * (dyz - dzy);
+      float vy = hy[p] - S * (dzx - dxz);
+      float vz = hz[p] - S * (dxy - dyx);
+      if (src && x == a.source_x) {
+        vy -= (sy * wave_y) * S;
+        vz += (sz * wave_z) * S;
+      }
+      H[i] = vx;
+      H[n + i] = vy;
+      H[2 * n + i] = vz;
     }
-    hx = hx - S * (dyz - dzy);
-    hy = hy - S * (dzx - dxz);
-    hz = hz - S * (dxy - dyx);
-    if (x == a.source_x) {
-      hy -= (a.src_e[2 * P + o] * wave_y) * S;
-      hz += (a.src_e[P + o] * wave_z) * S;
-    }
-    H[i] = hx;
-    H[n + i] = hy;
-    H[2 * n + i] = hz;
-    sx = px;
-    sy = py;
-    sz = pz;
   }
 }
 
@@ -388,16 +507,34 @@ int wfd_yee_step(const wfd_yee_plan* p, void* stream) {
   a.inv_nz = 1.0f / (float)p->nz;
   a.source_x = p->source_x;
   a.courant = p->courant;
-  a.x_chunk = p->x_chunk > 0 ? p->x_chunk : 4;
+  a.x_chunk = p->x_chunk > 0 ? p->x_chunk : 2;
+  WFD_REQUIRE(a.x_chunk == 1 || a.x_chunk == 2 || a.x_chunk == 4 || a.x_chunk == 8,
+              "yee_step: x_chunk must be 1, 2, 4 or 8");
   a.segs = (a.P + kBlock - 1) / kBlock;
   const int chunks = (a.nx + a.x_chunk - 1) / a.x_chunk;
   a.total = a.segs * chunks;
   a.per = (a.total + 7) / 8;
-  const int blocks = a.per * 8;
-  hipLaunchKernelGGL(k_yee_e, dim3(blocks), dim3(kBlock), 0, as_stream(stream), a);
-  WFD_LAUNCH_CHECK("yee_step(E)");
-  hipLaunchKernelGGL(k_yee_h, dim3(blocks), dim3(kBlock), 0, as_stream(stream), a);
-  WFD_LAUNCH_CHECK("yee_step(H)");
+  const dim3 grid(a.per * 8), block(kBlock);
+  hipStream_t s = as_stream(stream);
+  switch (a.x_chunk) {
+    case 1:
+      hipLaunchKernelGGL(k_yee_e<1>, grid, block, 0, s, a);
+      hipLaunchKernelGGL(k_yee_h<1>, grid, block, 0, s, a);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_yee_e<2>, grid, block, 0, s, a);
+      hipLaunchKernelGGL(k_yee_h<2>, grid, block, 0, s, a);
+      break;
+    case 4:
+      hipLaunchKernelGGL(k_yee_e<4>, grid, block, 0, s, a);
+      hipLaunchKernelGGL(k_yee_h<4>, grid, block, 0, s, a);
+      break;
+    default:
+      hipLaunchKernelGGL(k_yee_e<8>, grid, block, 0, s, a);
+      hipLaunchKernelGGL(k_yee_h<8>, grid, block, 0, s, a);
+      break;
+  }
+  WFD_LAUNCH_CHECK("yee_step");
   return WFD_OK;
 }
 

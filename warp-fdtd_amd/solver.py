@@ -348,11 +348,20 @@ class Solver:
     def _capture(self, length, record):
         return self._capture_fn(length, lambda: self.launch_step(record=record))
 
+    def tune(self):
+        """Pick the fused step's launch shape for this grid (autotune.py)."""
+        if self.cpml_storage == "full" and self.x_chunk == 0:
+            from .autotune import tune
+
+            self.x_chunk = tune(self)
+        return self.x_chunk
+
     def compile(self):
         """solver.py:239-250 — warm step, reset, capture graph_steps and tail."""
         start = time.perf_counter()
         self.launch_step()
         torch.cuda.synchronize(self.device)
+        self.tune()
         self.reset()
         self._destroy_graphs()
         for length in {self.graph_steps, self.steps % self.graph_steps} - {0}:
