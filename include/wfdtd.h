@@ -26,6 +26,7 @@ extern "C" {
 #define WFD_ERR_UNSUPPORTED (-3) /* valid but unsupported configuration -> ValueError */
 
 #define WFD_MAX_MONITORS 16
+#define WFD_MAX_STAGES 8
 
 /* ---------------------------------------------------------------- runtime */
 int wfd_version(void);
@@ -85,6 +86,15 @@ int wfd_advance(int* step, int delta, void* stream);
  * counter protocol needs no separate advance launch: counters[0] holds the
  * number of completed steps (the reference's `step`), counters[1] the step the
  * H half-step is applying (read by detectors recorded after the step). */
+/* A detector footprint the step stages every step: the raw E, H (after the
+ * step) and H_old (before it) of every cell of the half-open box, written to
+ * buf[(t % slots), 9, count] with component order Ex Ey Ez Hx Hy Hz Hox Hoy
+ * Hoz and cells in (x, y, z) order, z fastest (count = box volume). */
+typedef struct wfd_stage {
+  float* buf;
+  int box[6]; /* x0, x1, y0, y1, z0, z1 */
+} wfd_stage;
+
 typedef struct wfd_yee_plan {
   float* e;               /* (3, n) */
   float* h;               /* (3, n) */
@@ -107,6 +117,9 @@ typedef struct wfd_yee_plan {
   int source_x;           /* < 0: no source */
   float courant;
   int x_chunk;            /* planes per thread column; 0 picks a default */
+  int nstage;             /* footprints staged by the H half-step */
+  int slots;              /* ring length of every stage buffer */
+  wfd_stage stages[WFD_MAX_STAGES];
 } wfd_yee_plan;
 
 int wfd_yee_step(const wfd_yee_plan* plan, void* stream);
@@ -138,6 +151,34 @@ typedef struct wfd_monitor {
 int wfd_record_monitors(const float* e, const float* h, const float* prev,
                         const int* step, const wfd_monitor* monitors, int count, int nx,
                         int ny, int nz, void* stream);
+
+/* Deferred detector recording from staged footprints (wfd_stage): for the m
+ * steps t = last-m+1 .. last (last = step[0]) in order, every detector
+ * colocates its plane from the staged raw fields exactly as detectors.py:
+ * 26-62 does from the live arrays, then applies detectors.py:99-106 (kind 0:
+ * float32 phasor += w_t * v where indices[t] >= 0; kind 1: flux[indices[t]] +=
+ * sum(cross * area), reduced in a fixed order) or the FP64 multi-frequency
+ * DFT (kind 2: phasor (nf, 6, count, 2) += w[t, f] * v).  Bitwise the same
+ * values as recording every step. */
+typedef struct wfd_replay {
+  const float* stage;     /* the footprint's stage buffer */
+  int box[6];             /* its footprint box */
+  int x0, y0, z0, sx, sy, sz;  /* detector plane */
+  int kind;               /* 0 phasor, 1 flux, 2 dft */
+  int exact, axis;
+  int steps;              /* length of indices / weights */
+  int nf;                 /* kind 2: number of frequencies */
+  const int* indices;     /* kinds 0, 1 */
+  const float* weights;   /* kind 0: (steps, 2) */
+  const double* weights64;/* kind 2: (steps, nf, 2) */
+  float* phasor;          /* kind 0: (6, count, 2) */
+  double* phasor64;       /* kind 2: (nf, 6, count, 2) */
+  float* flux;            /* kind 1 */
+  const float* area;      /* kind 1: (count) */
+} wfd_replay;
+
+int wfd_replay_monitors(const wfd_replay* monitors, int count, int slots, const int* step,
+                        int m, int nx, int ny, int nz, void* stream);
 
 /* Multi-frequency running DFT (no reference counterpart; the single-frequency
  * case is detectors.py:65-106 with a per-frequency weight table).  stage writes

@@ -191,6 +191,25 @@ def test_packed_storage_equals_full(pkg, gpu):
     assert pkg.solver.owned_array_bytes(packed) > packed.E.capacity
 
 
+def test_staged_replay_equals_per_step_recording(pkg, gpu):
+    """hann_200fs: the fused path (footprints staged in the step kernel,
+    detectors replayed once per 64-step chunk) and the packed path (every
+    monitor recorded each step) give bitwise equal phasors and fluxes."""
+    runs = []
+    for storage in ("full", "packed"):
+        s = pkg.solver.Solver(BUNDLES / "hann_200fs", cpml_storage=storage)
+        pkg.detectors.attach(s)
+        s.compile()
+        s.reset()
+        s.run()
+        assert len(s.staged()) == (5 if storage == "full" else 0)
+        runs.append(s)
+    for a, b in zip(runs[0].monitors, runs[1].monitors):
+        np.testing.assert_array_equal(a.phasor.numpy(), b.phasor.numpy())
+        np.testing.assert_array_equal(a.flux.numpy(), b.flux.numpy())
+    np.testing.assert_array_equal(runs[0].E.numpy(), runs[1].E.numpy())
+
+
 def test_hann20_modal_phasors_and_transmission(pkg, gpu):
     """3M-cell bend, 5,245 steps, exact colocation + Hann-windowed modal
     phasors: against the Warp phasors and transmission (Warp vs FDTDX: T abs

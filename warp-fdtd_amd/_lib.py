@@ -25,6 +25,15 @@ L = ctypes.c_int64
 F = ctypes.c_float
 
 
+WFD_MAX_STAGES = 8
+
+
+class StageDesc(ctypes.Structure):
+    """wfd_stage (include/wfdtd.h)."""
+
+    _fields_ = [("buf", P), ("box", I * 6)]
+
+
 class YeePlan(ctypes.Structure):
     """wfd_yee_plan (include/wfdtd.h)."""
 
@@ -34,7 +43,18 @@ class YeePlan(ctypes.Structure):
         ("wave_e", P), ("wave_h", P), ("counters", P), ("hprev", P),
         ("hbox", I * 6), ("save_h_from", I), ("coeff_rows", I), ("steps", I),
         ("nx", I), ("ny", I), ("nz", I), ("source_x", I), ("courant", F),
-        ("x_chunk", I),
+        ("x_chunk", I), ("nstage", I), ("slots", I), ("stages", StageDesc * WFD_MAX_STAGES),
+    ]
+
+
+class ReplayDesc(ctypes.Structure):
+    """wfd_replay (include/wfdtd.h)."""
+
+    _fields_ = [
+        ("stage", P), ("box", I * 6), ("x0", I), ("y0", I), ("z0", I), ("sx", I), ("sy", I),
+        ("sz", I), ("kind", I), ("exact", I), ("axis", I), ("steps", I), ("nf", I),
+        ("indices", P), ("weights", P), ("weights64", P), ("phasor", P), ("phasor64", P),
+        ("flux", P), ("area", P),
     ]
 
 
@@ -84,6 +104,7 @@ SIGNATURES = {
     "wfd_yee_step": [ctypes.POINTER(YeePlan), P],
     "wfd_record": [P, P, P, P, P, P, P, P, P] + [I] * 13 + [P],
     "wfd_record_monitors": [P, P, P, P, ctypes.POINTER(MonitorDesc), I, I, I, I, P],
+    "wfd_replay_monitors": [ctypes.POINTER(ReplayDesc), I, I, P, I, I, I, I, P],
     "wfd_dft_stage": [P, P, P, P, P] + [I] * 11 + [P],
     "wfd_dft_accumulate": [P, I, P, I, I, P, I, P, I, P],
     "wfd_record_volume": [P, P, P, P, P, P, P, I, I, I, I, I, P],

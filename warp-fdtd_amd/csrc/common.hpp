@@ -52,6 +52,29 @@ struct F3 {
   float x, y, z;
 };
 
+// Device copy of a wfd_stage footprint (box plus its cell count).
+struct StageBox {
+  float* buf;
+  int x0, x1, y0, y1, z0, z1;
+  int count;
+};
+
+inline int to_stage_box(const wfd_stage& s, int nx, int ny, int nz, StageBox& out) {
+  const int* b = s.box;
+  if (!s.buf || b[0] < 0 || b[2] < 0 || b[4] < 0 || b[1] > nx || b[3] > ny || b[5] > nz ||
+      b[1] <= b[0] || b[3] <= b[2] || b[5] <= b[4])
+    return WFD_ERR_ARGUMENT;
+  out.buf = s.buf;
+  out.x0 = b[0];
+  out.x1 = b[1];
+  out.y0 = b[2];
+  out.y1 = b[3];
+  out.z0 = b[4];
+  out.z1 = b[5];
+  out.count = (b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]);
+  return WFD_OK;
+}
+
 // kernels.py:8-26 — zero-padded (or periodic) SoA fetch at (x*ny+y)*nz+z.
 __device__ __forceinline__ F3 sample(const float* __restrict__ f, int n, int x, int y,
                                      int z, int nx, int ny, int nz, int periodic) {

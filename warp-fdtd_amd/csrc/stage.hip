@@ -1,0 +1,272 @@
+// Deferred detector recording for gfx950.  The H half-step stages the raw E,
+// H and H_old of every detector footprint each step (yee.hip); once per graph
+// chunk one launch replays the steps in order, colocating each plane from the
+// staged values with exactly the arithmetic of detectors.py:26-62 and then
+// accumulating like detectors.py:99-106, so results equal per-step recording
+// bit for bit while the time loop carries no detector launches.
+#include "common.hpp"
+
+namespace wfd {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kFreqGroup = 8;
+
+struct Replay {
+  const float* stage;
+  int fx0, fy0, fz0, fsy, fsz, fcount;  // footprint origin, y/z extents, cells
+  int x0, y0, z0, sx, sy, sz, count;    // plane
+  int kind, exact, axis, steps, nf;
+  const int* indices;
+  const float* weights;
+  const double* weights64;
+  float* phasor;
+  double* phasor64;
+  float* flux;
+  const float* area;
+};
+
+struct ReplayArgs {
+  Replay r[WFD_MAX_STAGES];
+  int begin[WFD_MAX_STAGES + 1];
+  int count;
+};
+
+// Staged field f (0-2 E, 3-5 H, 6-8 H_old) at (x, y, z); zero outside the grid
+// (every in-grid cell the colocation touches lies in the footprint).
+struct Slot {
+  const float* base;
+  const Replay* r;
+  int nx, ny, nz;
+  __device__ __forceinline__ float at(int f, int x, int y, int z) const {
+    if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) return 0.f;
+    const int idx = ((x - r->fx0) * r->fsy + (y - r->fy0)) * r->fsz + (z - r->fz0);
+    return base[f * r->fcount + idx];
+  }
+  // detectors.py:10-23 for component c
+  __device__ __forceinline__ float ht(int c, int x, int y, int z) const {
+    return 0.5f * (at(3 + c, x, y, z) + at(6 + c, x, y, z));
+  }
+};
+
+// Component c (0-5: Ex Ey Ez Hx Hy Hz) of detectors.colocate at (x, y, z).
+__device__ __forceinline__ float colocated(const Slot& s, int c, int x, int y, int z,
+                                           int exact) {
+  if (!exact) return s.at(c, x, y, z);
+  switch (c) {
+    case 0:
+      return 0.25f * (((s.at(0, x, y, z) + s.at(0, x - 1, y, z)) + s.at(0, x, y, z + 1)) +
+                      s.at(0, x - 1, y, z + 1));
+    case 1:
+      return 0.25f * (((s.at(1, x, y, z) + s.at(1, x, y - 1, z)) + s.at(1, x, y, z + 1)) +
+                      s.at(1, x, y - 1, z + 1));
+    case 2:
+      return s.at(2, x, y, z);
+    case 3:
+      return 0.5f * (s.ht(0, x, y, z) + s.ht(0, x, y - 1, z));
+    case 4:
+      return 0.5f * (s.ht(1, x, y, z) + s.ht(1, x - 1, y, z));
+    default:
+      return 0.125f * (((((((s.ht(2, x, y, z) + s.ht(2, x - 1, y, z)) + s.ht(2, x, y - 1, z)) +
+                           s.ht(2, x - 1, y - 1, z)) +
+                          s.ht(2, x, y, z + 1)) +
+                         s.ht(2, x - 1, y, z + 1)) +
+                        s.ht(2, x, y - 1, z + 1)) +
+                       s.ht(2, x - 1, y - 1, z + 1));
+  }
+}
+
+__device__ __forceinline__ void plane_xyz(const Replay& r, int i, int& x, int& y, int& z) {
+  x = r.x0 + i / (r.sy * r.sz);
+  y = r.y0 + (i / r.sz) % r.sy;
+  z = r.z0 + i % r.sz;
+}
+
+__global__ __launch_bounds__(kBlock) void k_replay(ReplayArgs args, int slots,
+                                                   const int* __restrict__ step, int m,
+                                                   int nx, int ny, int nz) {
+  const int b = blockIdx.x;
+  int k = 0;
+  while (k + 1 < args.count && b >= args.begin[k + 1]) ++k;
+  const Replay& r = args.r[k];
+  const int last = step[0];
+  Slot s;
+  s.r = &r;
+  s.nx = nx;
+  s.ny = ny;
+  s.nz = nz;
+  const int local = b - args.begin[k];
+  if (r.kind == 1) {  // flux: one block, plane reduced per step in a fixed order
+    __shared__ float part[kBlock];
+    for (int j = 0; j < m; ++j) {
+      const int t = last - m + 1 + j;
+      if (t < 0 || t >= r.steps) continue;
+      const int index = r.indices[t];
+      if (index < 0) continue;
+      s.base = r.stage + (size_t)(t % slots) * 9 * r.fcount;
+      float acc = 0.f;
+      for (int i = threadIdx.x; i < r.count; i += kBlock) {
+        int x, y, z;
+        plane_xyz(r, i, x, y, z);
+        float e[3], h[3];
+        for (int c = 0; c < 3; ++c) {
+          e[c] = colocated(s, c, x, y, z, r.exact);
+          h[c] = colocated(s, 3 + c, x, y, z, r.exact);
+        }
+        float cr;
+        if (r.axis == 0) cr = e[1] * h[2] - e[2] * h[1];
+        else if (r.axis == 1) cr = e[2] * h[0] - e[0] * h[2];
+        else cr = e[0] * h[1] - e[1] * h[0];
+        acc += cr * r.area[i];
+      }
+      part[threadIdx.x] = acc;
+      __syncthreads();
+      for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) r.flux[index] += part[0];
+      __syncthreads();
+    }
+    return;
+  }
+  if (r.kind == 0) {  // float32 phasor, one thread per plane cell
+    const int i = local * kBlock + threadIdx.x;
+    if (i >= r.count) return;
+    int x, y, z;
+    plane_xyz(r, i, x, y, z);
+    float2 acc[6];
+    float2* ph = reinterpret_cast<float2*>(r.phasor);
+    for (int c = 0; c < 6; ++c) acc[c] = ph[c * r.count + i];
+    for (int j = 0; j < m; ++j) {
+      const int t = last - m + 1 + j;
+      if (t < 0 || t >= r.steps) continue;
+      if (r.indices[t] < 0) continue;
+      s.base = r.stage + (size_t)(t % slots) * 9 * r.fcount;
+      const float wr = r.weights[2 * t], wi = r.weights[2 * t + 1];
+      for (int c = 0; c < 6; ++c) {
+        const float v = colocated(s, c, x, y, z, r.exact);
+        acc[c].x += wr * v;
+        acc[c].y += wi * v;
+      }
+    }
+    for (int c = 0; c < 6; ++c) ph[c * r.count + i] = acc[c];
+    return;
+  }
+  // kind 2: FP64 multi-frequency DFT; one thread per (component, cell), frequency
+  // groups of 8 along the block index.
+  const int groups = (r.nf + kFreqGroup - 1) / kFreqGroup;
+  const int per_group = (6 * r.count + kBlock - 1) / kBlock;
+  const int g = local / per_group;
+  const int ci = (local - g * per_group) * kBlock + threadIdx.x;
+  if (g >= groups || ci >= 6 * r.count) return;
+  const int c = ci / r.count, i = ci - c * r.count;
+  const int f0 = g * kFreqGroup;
+  int x, y, z;
+  plane_xyz(r, i, x, y, z);
+  double re[kFreqGroup], im[kFreqGroup];
+#pragma unroll
+  for (int q = 0; q < kFreqGroup; ++q) re[q] = im[q] = 0.0;
+  for (int j = 0; j < m; ++j) {
+    const int t = last - m + 1 + j;
+    if (t < 0 || t >= r.steps) continue;
+    s.base = r.stage + (size_t)(t % slots) * 9 * r.fcount;
+    const double v = (double)colocated(s, c, x, y, z, r.exact);
+    const double* w = r.weights64 + 2 * ((size_t)t * r.nf + f0);
+#pragma unroll
+    for (int q = 0; q < kFreqGroup; ++q) {
+      if (f0 + q < r.nf) {
+        re[q] += w[2 * q] * v;
+        im[q] += w[2 * q + 1] * v;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kFreqGroup; ++q) {
+    if (f0 + q < r.nf) {
+      double2* p = reinterpret_cast<double2*>(r.phasor64) + ((size_t)(f0 + q) * 6 * r.count + ci);
+      double2 cur = *p;
+      cur.x += re[q];
+      cur.y += im[q];
+      *p = cur;
+    }
+  }
+}
+
+}  // namespace
+}  // namespace wfd
+
+using namespace wfd;
+
+extern "C" {
+
+int wfd_replay_monitors(const wfd_replay* monitors, int count, int slots, const int* step,
+                        int m, int nx, int ny, int nz, void* stream) {
+  WFD_REQUIRE(monitors && step && count >= 1 && count <= WFD_MAX_STAGES,
+              "replay: 1..%d detectors per launch", WFD_MAX_STAGES);
+  WFD_REQUIRE(slots > 0 && m >= 0 && m <= slots, "replay: need 0 <= m <= slots");
+  WFD_REQUIRE(nx > 0 && ny > 0 && nz > 0 && cells(nx, ny, nz) <= kMaxCells, "replay: bad grid");
+  if (m == 0) return WFD_OK;
+  ReplayArgs args;
+  args.count = count;
+  int blocks = 0;
+  for (int k = 0; k < count; ++k) {
+    const wfd_replay& d = monitors[k];
+    Replay& r = args.r[k];
+    const int* f = d.box;
+    WFD_REQUIRE(d.stage && f[0] >= 0 && f[2] >= 0 && f[4] >= 0 && f[1] <= nx && f[3] <= ny &&
+                    f[5] <= nz && f[1] > f[0] && f[3] > f[2] && f[5] > f[4],
+                "replay: detector %d footprint outside the grid", k);
+    WFD_REQUIRE(d.sx > 0 && d.sy > 0 && d.sz > 0 && d.steps > 0, "replay: detector %d plane", k);
+    // every in-grid cell the colocation stencil reads must lie in the footprint
+    const int lox = d.exact ? d.x0 - 1 : d.x0, loy = d.exact ? d.y0 - 1 : d.y0;
+    const int hiz = d.exact ? d.z0 + d.sz + 1 : d.z0 + d.sz;
+    WFD_REQUIRE(f[0] <= (lox < 0 ? 0 : lox) && f[2] <= (loy < 0 ? 0 : loy) && f[4] <= d.z0 &&
+                    f[1] >= d.x0 + d.sx && f[3] >= d.y0 + d.sy && f[5] >= (hiz > nz ? nz : hiz),
+                "replay: detector %d footprint does not cover its stencil", k);
+    WFD_REQUIRE(d.kind >= 0 && d.kind <= 2, "replay: detector %d kind", k);
+    WFD_REQUIRE(d.kind == 2 || d.indices, "replay: detector %d without indices", k);
+    WFD_REQUIRE(d.kind != 0 || (d.weights && d.phasor), "replay: phasor detector %d arrays", k);
+    WFD_REQUIRE(d.kind != 1 || (d.flux && d.area && d.axis >= 0 && d.axis < 3),
+                "replay: flux detector %d arrays", k);
+    WFD_REQUIRE(d.kind != 2 || (d.weights64 && d.phasor64 && d.nf > 0),
+                "replay: dft detector %d arrays", k);
+    r.stage = d.stage;
+    r.fx0 = f[0];
+    r.fy0 = f[2];
+    r.fz0 = f[4];
+    r.fsy = f[3] - f[2];
+    r.fsz = f[5] - f[4];
+    r.fcount = (f[1] - f[0]) * r.fsy * r.fsz;
+    r.x0 = d.x0;
+    r.y0 = d.y0;
+    r.z0 = d.z0;
+    r.sx = d.sx;
+    r.sy = d.sy;
+    r.sz = d.sz;
+    r.count = d.sx * d.sy * d.sz;
+    r.kind = d.kind;
+    r.exact = d.exact;
+    r.axis = d.axis;
+    r.steps = d.steps;
+    r.nf = d.nf;
+    r.indices = d.indices;
+    r.weights = d.weights;
+    r.weights64 = d.weights64;
+    r.phasor = d.phasor;
+    r.phasor64 = d.phasor64;
+    r.flux = d.flux;
+    r.area = d.area;
+    args.begin[k] = blocks;
+    if (d.kind == 1) blocks += 1;
+    else if (d.kind == 0) blocks += (r.count + kBlock - 1) / kBlock;
+    else blocks += ((6 * r.count + kBlock - 1) / kBlock) * ((d.nf + kFreqGroup - 1) / kFreqGroup);
+  }
+  args.begin[count] = blocks;
+  hipLaunchKernelGGL(k_replay, dim3(blocks), dim3(kBlock), 0, as_stream(stream), args, slots,
+                     step, m, nx, ny, nz);
+  WFD_LAUNCH_CHECK("replay");
+  return WFD_OK;
+}
+
+}  // extern "C"

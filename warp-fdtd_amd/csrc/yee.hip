@@ -145,7 +145,34 @@ struct YeeArgs {
   int segs;   // plane segments per x chunk
   int total;  // logical blocks
   int per;    // logical blocks per XCD stripe
+  int nstage;
+  int slots;
+  StageBox st[WFD_MAX_STAGES];
 };
+
+// Stage E (after the step), H (after) and H_old of a footprint cell.
+__device__ __forceinline__ void stage_cell(const StageBox* st, int nstage, int slot, int x,
+                                           int y, int z, float ex, float ey, float ez,
+                                           float hx, float hy, float hz, float ox, float oy,
+                                           float oz) {
+  for (int b = 0; b < nstage; ++b) {
+    const StageBox& s = st[b];
+    if (x >= s.x0 && x < s.x1 && y >= s.y0 && y < s.y1 && z >= s.z0 && z < s.z1) {
+      const int cnt = s.count;
+      float* d = s.buf + (size_t)slot * 9 * cnt +
+                 ((x - s.x0) * (s.y1 - s.y0) + (y - s.y0)) * (s.z1 - s.z0) + (z - s.z0);
+      d[0] = ex;
+      d[cnt] = ey;
+      d[2 * cnt] = ez;
+      d[3 * cnt] = hx;
+      d[4 * cnt] = hy;
+      d[5 * cnt] = hz;
+      d[6 * cnt] = ox;
+      d[7 * cnt] = oy;
+      d[8 * cnt] = oz;
+    }
+  }
+}
 
 struct Cell {
   int o, y, z, x0, nvalid;
@@ -407,6 +434,9 @@ __global__ __launch_bounds__(kBlock) void k_yee_h(YeeArgs a) {
       H[i] = vx;
       H[n + i] = vy;
       H[2 * n + i] = vz;
+      if (a.nstage)
+        stage_cell(a.st, a.nstage, t % a.slots, x, y, z, ex[p], ey[p], ez[p], vx, vy, vz, hx[p],
+                   hy[p], hz[p]);
     }
   }
 }
@@ -510,6 +540,14 @@ int wfd_yee_step(const wfd_yee_plan* p, void* stream) {
   a.x_chunk = p->x_chunk > 0 ? p->x_chunk : 2;
   WFD_REQUIRE(a.x_chunk == 1 || a.x_chunk == 2 || a.x_chunk == 4 || a.x_chunk == 8,
               "yee_step: x_chunk must be 1, 2, 4 or 8");
+  WFD_REQUIRE(p->nstage >= 0 && p->nstage <= WFD_MAX_STAGES, "yee_step: 0..%d stages",
+              WFD_MAX_STAGES);
+  WFD_REQUIRE(p->nstage == 0 || p->slots > 0, "yee_step: stage slots must be positive");
+  a.nstage = p->nstage;
+  a.slots = p->slots > 0 ? p->slots : 1;
+  for (int b = 0; b < p->nstage; ++b)
+    WFD_REQUIRE(to_stage_box(p->stages[b], p->nx, p->ny, p->nz, a.st[b]) == WFD_OK,
+                "yee_step: stage %d outside the grid or without a buffer", b);
   a.segs = (a.P + kBlock - 1) / kBlock;
   const int chunks = (a.nx + a.x_chunk - 1) / a.x_chunk;
   a.total = a.segs * chunks;

@@ -1,15 +1,18 @@
 """Colocated field/flux detectors on libwfdtd (drop-in for warp_fdtd/detectors.py).
 
-Time accumulation runs on the device: every attached Monitor of a solver is
-recorded by one kernel launch per step (``record_all``); ``record`` is the
-single-monitor kernel with the reference's argument order.
+Time accumulation runs on the device.  On the fused path the step kernel
+stages each detector's raw footprint every step and one replay launch per
+graph chunk records all detectors in step order (bitwise the per-step
+result); otherwise every Monitor is recorded by one launch per step
+(``record_all``).  ``record`` is the single-monitor kernel with the
+reference's argument order.
 """
 
 import ctypes
 
 import numpy as np
 
-from ._lib import MonitorDesc, WFD_MAX_MONITORS, call, ptr, stream
+from ._lib import MonitorDesc, ReplayDesc, WFD_MAX_MONITORS, call, ptr, stream
 from .array import Array
 from .kernels import Kernel
 
@@ -79,6 +82,30 @@ class Monitor:
     def footprint(self):
         return colocation_footprint(self.obj["grid_slice"], self.shape_of_grid)
 
+    def stage_box(self):
+        """Cells whose raw fields the step stages for this monitor."""
+        if self.exact:
+            return self.footprint()
+        (x0, x1), (y0, y1), (z0, z1) = self.obj["grid_slice"]
+        return (x0, x1, y0, y1, z0, z1)
+
+    def replay_desc(self):
+        d = ReplayDesc()
+        d.stage = ptr(self.stage)
+        for a, v in enumerate(self.stage_box()):
+            d.box[a] = int(v)
+        d.x0, d.y0, d.z0 = [a for a, b in self.obj["grid_slice"]]
+        d.sx, d.sy, d.sz = self.shape
+        d.kind = 0 if self.is_phasor else 1
+        d.exact, d.axis = int(self.exact), int(self.axis)
+        d.steps = int(len(self.indices_numpy))
+        d.indices = ptr(self.indices)
+        d.weights = ptr(self.weights)
+        d.phasor = ptr(self.phasor)
+        d.flux = ptr(self.flux)
+        d.area = ptr(self.area)
+        return d
+
     def reset(self):
         self.phasor.zero_()
         self.flux.zero_()
@@ -127,9 +154,9 @@ class DFTMonitor:
     accumulated in float64 (no reference counterpart; one wavelength with the
     reference's weights is the PhasorDetector of detectors.py:99-103).
 
-    Every step stages the plane's six colocated components; every graph chunk
-    (or every step when stepping eagerly) folds the staged steps into the
-    accumulators: phasor[f, c, i] += sum_t w[t, f] * field_c(t, i) with
+    The fused step stages the plane's raw footprint every step and one replay
+    per graph chunk colocates and folds the staged steps into the accumulators:
+    phasor[f, c, i] += sum_t w[t, f] * field_c(t, i) with
     w[t, f] = exp(i 2 pi f t dt) dt.
     """
 
@@ -153,8 +180,8 @@ class DFTMonitor:
         w = np.exp(2j * np.pi * self.frequencies[None, :] * t[:, None]) * self.dt
         self.weights = Array.from_numpy(np.stack((w.real, w.imag), axis=-1), np.float64,
                                         solver.device)
-        self.slots = int(solver.graph_steps)
-        self.stage = Array.zeros((self.slots, 6, self.n), device=solver.device)
+        self.stage = None
+        self._colocated = None
         self.phasor = Array.zeros((self.nf, 6, self.n, 2), np.float64, solver.device)
         self.shape_of_grid = solver.shape
         self.data = solver.data
@@ -167,8 +194,6 @@ class DFTMonitor:
             sh[a] = len(v)
             widths.append(v.reshape(sh))
         self.area_numpy = np.broadcast_to(widths[0] * widths[1] * widths[2], self.shape).copy()
-        self._flush_every = 1
-        self._pending = 0
 
     def active_steps(self, steps):
         return np.ones(steps, bool)
@@ -176,30 +201,40 @@ class DFTMonitor:
     def footprint(self):
         return colocation_footprint(self.obj["grid_slice"], self.shape_of_grid)
 
+    def stage_box(self):
+        if self.exact:
+            return self.footprint()
+        (x0, x1), (y0, y1), (z0, z1) = self.obj["grid_slice"]
+        return (x0, x1, y0, y1, z0, z1)
+
+    def replay_desc(self):
+        d = ReplayDesc()
+        d.stage = ptr(self.stage)
+        for a, v in enumerate(self.stage_box()):
+            d.box[a] = int(v)
+        d.x0, d.y0, d.z0 = self.lo
+        d.sx, d.sy, d.sz = self.shape
+        d.kind = 2
+        d.exact, d.axis = int(self.exact), int(self.axis)
+        d.steps = int(self.steps)
+        d.nf = int(self.nf)
+        d.weights64 = ptr(self.weights)
+        d.phasor64 = ptr(self.phasor)
+        return d
+
     def reset(self):
         self.phasor.zero_()
-        self._pending = 0
-
-    def begin_chunk(self, length):
-        if length > self.slots:
-            raise ValueError("DFT staging holds at most graph_steps steps")
-        self._flush_every = length
-        self._pending = 0
-
-    def end_chunk(self):
-        self._flush_every = 1
-        self._pending = 0
 
     def launch(self, s):
+        """Per-step recording (used when the solver does not stage, e.g. the
+        packed-CPML kernels): colocate into a one-slot buffer, accumulate."""
+        if self._colocated is None:
+            self._colocated = Array.zeros((1, 6, self.n), device=s.device)
         call("wfd_dft_stage", ptr(s.E), ptr(s.H), ptr(s.hprev), ptr(s.record_step),
-             ptr(self.stage), self.slots, *s.shape, *self.lo, *self.shape, int(self.exact),
+             ptr(self._colocated), 1, *s.shape, *self.lo, *self.shape, int(self.exact),
              stream())
-        self._pending += 1
-        if self._pending == self._flush_every:
-            call("wfd_dft_accumulate", ptr(self.stage), self.slots, ptr(self.weights),
-                 self.steps, self.nf, ptr(s.record_step), self._pending, ptr(self.phasor),
-                 self.n, stream())
-            self._pending = 0
+        call("wfd_dft_accumulate", ptr(self._colocated), 1, ptr(self.weights), self.steps,
+             self.nf, ptr(s.record_step), 1, ptr(self.phasor), self.n, stream())
 
     def phasors(self):
         raw = self.phasor.numpy()

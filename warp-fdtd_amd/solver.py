@@ -17,7 +17,7 @@ import numpy as np
 import torch
 
 from . import kernels as k
-from ._lib import YeePlan, call, ptr, stream
+from ._lib import ReplayDesc, WFD_MAX_STAGES, YeePlan, call, ptr, stream
 from .array import Array
 
 
@@ -232,6 +232,8 @@ class Solver:
         self.graphs = {}
         self._quiet_graphs = {}
         self._capture_stream = None
+        self._flush_every = 1
+        self._pending = 0
 
     # ------------------------------------------------------------ stepping
     @property
@@ -239,18 +241,35 @@ class Solver:
         """Device counter holding the step just applied, as monitors read it."""
         return self.counters[1:2] if self.cpml_storage == "full" else self.counters[0:1]
 
+    def staged(self):
+        """Monitors recorded through footprint staging + chunk replay (fused
+        path only; at most WFD_MAX_STAGES); the rest record every step."""
+        if self.cpml_storage != "full":
+            return []
+        return [m for m in self.monitors if hasattr(m, "replay_desc")][:WFD_MAX_STAGES]
+
+    def _stage_buffers(self):
+        for m in self.staged():
+            box = m.stage_box()
+            count = (box[1] - box[0]) * (box[3] - box[2]) * (box[5] - box[4])
+            buf = getattr(m, "stage", None)
+            if buf is None or buf.shape != (self.graph_steps, 9, count):
+                m.stage = Array.zeros((self.graph_steps, 9, count), device=self.device)
+
     def hprev_box(self):
-        """Half-open (x0, x1, y0, y1, z0, z1) of cells whose hprev is read: the
-        union of the exact monitors' footprints (a monitor without a
-        ``footprint`` reads the whole volume)."""
+        """Half-open (x0, x1, y0, y1, z0, z1) of cells whose hprev is read by
+        monitors recorded every step: the union of their footprints (one
+        without a ``footprint`` reads the whole volume).  Staged monitors take
+        H_old from their stage buffers and need none."""
         nx, ny, nz = self.shape
         full = (0, nx, 0, ny, 0, nz)
         if self.full_hprev:
             return full
+        staged = set(map(id, self.staged()))
         lo = [nx, ny, nz]
         hi = [0, 0, 0]
         for m in self.monitors:
-            if not getattr(m, "exact", True):
+            if id(m) in staged or not getattr(m, "exact", True):
                 continue
             box = m.footprint() if hasattr(m, "footprint") else None
             if box is None:
@@ -262,7 +281,7 @@ class Solver:
             return (0, 0, 0, 0, 0, 0)
         return (lo[0], hi[0], lo[1], hi[1], lo[2], hi[2])
 
-    def _plan(self):
+    def _plan(self, stage=True):
         p = YeePlan()
         p.e, p.h = ptr(self.E), ptr(self.H)
         p.inv_eps = ptr(self.eps)
@@ -281,6 +300,15 @@ class Solver:
         p.source_x = int(self.source_x)
         p.courant = float(self.courant)
         p.x_chunk = self.x_chunk
+        p.slots = int(self.graph_steps)
+        if stage:
+            self._stage_buffers()
+            staged = self.staged()
+            p.nstage = len(staged)
+            for b, m in enumerate(staged):
+                p.stages[b].buf = ptr(m.stage)
+                for a, v in enumerate(m.stage_box()):
+                    p.stages[b].box[a] = int(v)
         return p
 
     def reset(self):
@@ -288,11 +316,12 @@ class Solver:
             a.zero_()
         for monitor in self.monitors:
             monitor.reset()
+        self._pending = 0
         torch.cuda.synchronize(self.device)
 
-    def _step_fields(self):
+    def _step_fields(self, stage=True):
         if self.cpml_storage == "full":
-            k.yee_step(self._plan())
+            k.yee_step(self._plan(stage))
             return
         common = [self.hprev, self.save_h_from, *self.shape, self.source_x, self.courant, 0]
         k.update_e_packed(self.E, self.H, self.eps, self.psi_e, self.coeffs["E"],
@@ -300,15 +329,30 @@ class Solver:
         k.update_h_packed(self.H, self.E, self.eps, self.psi_h, self.coeffs["H"],
                           self.sources["E"], self.waves["E"], self.step, *common)
 
+    def _replay(self, m_steps):
+        staged = self.staged()
+        if not staged or m_steps == 0:
+            return
+        descs = (ReplayDesc * len(staged))(*[m.replay_desc() for m in staged])
+        call("wfd_replay_monitors", descs, len(staged), int(self.graph_steps),
+             ptr(self.record_step), int(m_steps), *self.shape, stream())
+
     def _record(self):
+        """Monitors recorded every step; staged ones replay once per chunk."""
         from .detectors import Monitor, record_all
 
-        plain = [m for m in self.monitors if isinstance(m, Monitor)]
+        staged = set(map(id, self.staged()))
+        plain = [m for m in self.monitors if isinstance(m, Monitor) and id(m) not in staged]
         if plain:
             record_all(self, plain)
         for m in self.monitors:
-            if not isinstance(m, Monitor):
+            if not isinstance(m, Monitor) and id(m) not in staged:
                 m.launch(self)
+        if staged:
+            self._pending += 1
+            if self._pending == self._flush_every:
+                self._replay(self._pending)
+                self._pending = 0
 
     def _finish_step(self):
         if self.cpml_storage != "full":
@@ -316,22 +360,21 @@ class Solver:
 
     def launch_step(self, record=True):
         """solver.py:194-237 — E, H, monitors, time advance."""
-        self._step_fields()
+        self._step_fields(stage=record)
         if record:
             self._record()
         self._finish_step()
 
     def _capture_fn(self, length, fn):
-        """HIP graph of `length` calls of fn, captured on a library stream."""
+        """HIP graph of `length` calls of fn, captured on a library stream;
+        staged monitors replay once at the end of the chunk."""
         if self._capture_stream is None:
             handle = ctypes.c_void_p()
             call("wfd_stream_create", ctypes.byref(handle))
             self._capture_stream = handle
         s = self._capture_stream
         torch.cuda.synchronize(self.device)
-        chunked = [m for m in self.monitors if hasattr(m, "begin_chunk")]
-        for m in chunked:
-            m.begin_chunk(length)
+        self._flush_every, self._pending = length, 0
         call("wfd_graph_begin", s)
         ext = torch.cuda.ExternalStream(s.value, device=self.device)
         try:
@@ -341,8 +384,7 @@ class Solver:
         finally:
             graph = ctypes.c_void_p()
             call("wfd_graph_end", s, ctypes.byref(graph))
-            for m in chunked:
-                m.end_chunk()
+            self._flush_every, self._pending = 1, 0
         return graph
 
     def _capture(self, length, record):
