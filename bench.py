@@ -51,11 +51,23 @@ def build_case():
     return pkg, path
 
 
-def kernel_time_per_step(pkg, s, repeats):
-    """Average duration of the fused Yee step (E + H kernel pair) from HIP
-    events recorded on the launch stream around each launch."""
+def kernel_time_per_step(pkg, s, repeats, chunk=64):
+    """Average duration of one Yee step from HIP events recorded on the launch
+    stream: around each fused E + H kernel pair, or around resident launches
+    of `chunk` steps (divided by chunk)."""
     import torch
 
+    if s.resident:
+        pairs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                 for _ in range(max(4, repeats // 8))]
+        s.resident_steps(chunk, record=False)
+        for a, b in pairs:
+            a.record()
+            s.resident_steps(chunk, record=False)
+            b.record()
+        s.check()
+        times = np.array([a.elapsed_time(b) for a, b in pairs]) * 1e-3 / chunk
+        return float(np.median(times)), float(np.mean(times))
     plan = s._plan()
     pairs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
              for _ in range(repeats)]
@@ -120,6 +132,8 @@ def main():
     p.add_argument("--kernel-repeats", type=int, default=200)
     p.add_argument("--x-chunk", type=int, default=0,
                    help="planes per thread column of the fused step (0: autotune)")
+    p.add_argument("--resident", choices=("auto", "on", "off"), default="auto",
+                   help="on-chip-resident stepping (auto: the faster of the two)")
     args = p.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build "
@@ -130,7 +144,8 @@ def main():
     import torch
 
     pkg, case = build_case()
-    s = pkg.solver.Solver(case, x_chunk=args.x_chunk)
+    resident = {"auto": "auto", "on": True, "off": False}[args.resident]
+    s = pkg.solver.Solver(case, x_chunk=args.x_chunk, resident=resident)
     pkg.detectors.attach(s)
     s.compile()
     if args.warmup:
@@ -146,6 +161,7 @@ def main():
     end_ev.record()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    s.check()
     gpu_elapsed = start_ev.elapsed_time(end_ev) * 1e-3
     cells = s.n
     value = cells * args.steps / elapsed / 1e6
@@ -156,13 +172,18 @@ def main():
     achieved = cells * B / mean_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_yee_e + k_yee_h (one fused step)",
+                "kernel": ("k_resident (steps per launch: 64)" if s.resident
+                           else "k_yee_e + k_yee_h (one fused step)"),
                 "bytes_per_cell_update": round(B, 2), "f_cpml": round(f_cpml, 4),
                 "kernel_us_per_step_mean": round(mean_s * 1e6, 3),
                 "kernel_us_per_step_median": round(median_s * 1e6, 3),
-                "x_chunk": s.x_chunk}
+                "x_chunk": s.x_chunk, "resident": bool(s.resident)}
+    if s.resident:
+        L = s.layout
+        roofline["resident_layout"] = {"sub_box": [L.sx, L.sy, L.sz],
+                                       "workgroups": [L.gx, L.gy, L.gz]}
     line = {
-        "metric": "Mcell-updates/s (fused Yee+CPML step, config-2 waveguide)",
+        "metric": "Mcell-updates/s + % HBM roofline (fused Yee+CPML step, 160x96x80 waveguide)",
         "value": round(value, 1),
         "unit": "Mcell-updates/s",
         "n_gpus": 1,

@@ -124,6 +124,30 @@ typedef struct wfd_yee_plan {
 
 int wfd_yee_step(const wfd_yee_plan* plan, void* stream);
 
+/* ------------------------------------------------- on-chip-resident stepping
+ * The same step as wfd_yee_step (bitwise), for grids whose E, H and 1/eps fit
+ * in the LDS of co-resident workgroups: sub-boxes of sx x sy x sz cells (one
+ * 1024-thread workgroup each, 152 KB of LDS) stay on chip for nsteps steps;
+ * faces move between workgroups through the workspace under per-workgroup
+ * step flags.  layout_for picks the sub-box when sx/sy/sz are 0.
+ * One cooperative launch per call (replaces nsteps pairs of half-step
+ * launches of solver.py:203-234); counters advance by nsteps afterwards.
+ * layout_for validates the decomposition (WFD_ERR_UNSUPPORTED when the
+ * workgroups cannot all be co-resident) and sizes the workspace, which the
+ * caller allocates zeroed; its last ints hold an error word (nonzero after a
+ * bounded wait timed out) that the caller reads after the run. */
+typedef struct wfd_resident_layout {
+  int sx, sy, sz;        /* sub-box cells along x, y, z */
+  int gx, gy, gz;        /* workgroups along each axis */
+  int blocks, capacity;  /* workgroups used / co-resident maximum */
+  int64_t workspace_bytes;
+} wfd_resident_layout;
+
+int wfd_resident_layout_for(const wfd_yee_plan* plan, int sx, int sy, int sz,
+                            wfd_resident_layout* layout);
+int wfd_resident_run(const wfd_yee_plan* plan, const wfd_resident_layout* layout,
+                     void* workspace, int nsteps, int spin_limit, void* stream);
+
 /* ------------------------------------------------------------- detectors
  * warp_fdtd/detectors.py:65-106 record(): one monitor, one launch over its
  * plane.  weights (steps, 2) float32 complex; phasor (6, count, 2) float32;

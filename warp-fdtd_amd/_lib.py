@@ -47,6 +47,15 @@ class YeePlan(ctypes.Structure):
     ]
 
 
+class ResidentLayout(ctypes.Structure):
+    """wfd_resident_layout (include/wfdtd.h)."""
+
+    _fields_ = [
+        ("sx", I), ("sy", I), ("sz", I), ("gx", I), ("gy", I), ("gz", I),
+        ("blocks", I), ("capacity", I), ("workspace_bytes", L),
+    ]
+
+
 class ReplayDesc(ctypes.Structure):
     """wfd_replay (include/wfdtd.h)."""
 
@@ -102,6 +111,8 @@ SIGNATURES = {
     "wfd_update_h_packed": _UPDATE,
     "wfd_advance": [P, I, P],
     "wfd_yee_step": [ctypes.POINTER(YeePlan), P],
+    "wfd_resident_layout_for": [ctypes.POINTER(YeePlan), I, I, I, ctypes.POINTER(ResidentLayout)],
+    "wfd_resident_run": [ctypes.POINTER(YeePlan), ctypes.POINTER(ResidentLayout), P, I, I, P],
     "wfd_record": [P, P, P, P, P, P, P, P, P] + [I] * 13 + [P],
     "wfd_record_monitors": [P, P, P, P, ctypes.POINTER(MonitorDesc), I, I, I, I, P],
     "wfd_replay_monitors": [ctypes.POINTER(ReplayDesc), I, I, P, I, I, I, I, P],

@@ -166,9 +166,10 @@ class Adjoint(Solver):
                 self._recompute_graphs[length] = self._capture_fn(
                     length, lambda: self.launch_step(record=False, curl=True))
         self.reset()
+        self._compiled = True
         return time.perf_counter() - start
 
-    def _replay(self, t0, t1, graphs_by_len, quiet_by_len, active):
+    def _replay_graphs(self, t0, t1, graphs_by_len, quiet_by_len, active):
         """Replay graphs covering steps [t0, t1) in chunks of the largest length."""
         s = stream()
         t = t0
@@ -191,7 +192,7 @@ class Adjoint(Solver):
                 dst.tensor.copy_(src.tensor)
             t1 = min(t0 + K, self.steps)
             graphs = {L: g for L, g in self.graphs.items() if L in (K, self.steps % K)}
-            self._replay(t0, t1, graphs, self._quiet_graphs, active)
+            self._replay_graphs(t0, t1, graphs, self._quiet_graphs, active)
         torch.cuda.synchronize(self.device)
 
     # --------------------------------------------------------- reverse

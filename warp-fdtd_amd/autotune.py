@@ -1,10 +1,12 @@
-"""Launch-shape autotuner for the fused Yee step.
+"""Launch-shape autotuner for the Yee time step.
 
-The only free launch parameter of wfd_yee_step is the x chunk a thread column
+The free launch parameter of wfd_yee_step is the x chunk a thread column
 marches (1, 2, 4 or 8 planes, compile-time unrolled).  ``tune`` times every
 candidate on the solver's own grid and buffers with HIP events and returns
-the fastest; results are cached in memory and in a JSON file keyed by grid
-shape, CPML rows and device name, so a shape is tuned once per machine.
+the fastest; ``tune_resident`` times the on-chip-resident kernel
+(wfd_resident_run) against the tuned fused step.  Results are cached in
+memory and in a JSON file keyed by grid shape, CPML rows and device name, so
+a shape is tuned once per machine.
 """
 
 import ctypes
@@ -81,5 +83,44 @@ def tune(solver, use_cache=True):
     _memory[key] = best
     table = _load()
     table[key] = {"x_chunk": best, "us": {str(k): round(v, 3) for k, v in timings.items()}}
+    _store(table)
+    return best
+
+
+def measure_resident(solver, layout, steps=64, repeats=5):
+    """Median microseconds per step of resident launches of `steps` steps
+    (monitors not recorded); leaves the solver in streaming mode."""
+    solver.use_resident(layout)
+    try:
+        solver.resident_steps(steps, record=False)
+        events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                  for _ in range(repeats)]
+        for a, b in events:
+            a.record()
+            solver.resident_steps(steps, record=False)
+            b.record()
+        solver.check()
+    finally:
+        solver.resident = False
+    return float(np.median([a.elapsed_time(b) for a, b in events]) * 1e3 / steps)
+
+
+def tune_resident(solver, layout, use_cache=True):
+    """True when the resident kernel beats the tuned fused step on this grid."""
+    key = _key(solver) + f"|resident|{layout.sx}x{layout.sy}x{layout.sz}"
+    if use_cache:
+        if key in _memory:
+            return _memory[key]
+        table = _load()
+        if key in table:
+            _memory[key] = bool(table[key]["resident"])
+            return _memory[key]
+    fused = measure(solver, solver.x_chunk or 2)
+    resident = measure_resident(solver, layout)
+    best = resident < fused
+    _memory[key] = best
+    table = _load()
+    table[key] = {"resident": best, "us": {"fused": round(fused, 3),
+                                           "resident": round(resident, 3)}}
     _store(table)
     return best

@@ -1,0 +1,543 @@
+// On-chip-resident Yee stepping for gfx950 (grids whose E, H and 1/eps fit
+// in the LDS of the co-resident workgroups: 152 KB x one workgroup per CU).
+//
+// The grid is cut into sub-boxes of sx x sy x sz cells (at most kResidentCells),
+// one 1024-thread workgroup each, which holds the sub-box's E, H and 1/eps in
+// LDS for a whole chunk of steps.  A half-step reads the stencil neighbours
+// from LDS and the sub-box faces of neighbouring workgroups from global face
+// buffers, guarded by per-workgroup step flags (agent-scope release/acquire).
+// Inside the chunk only CPML psi (slab cells), the faces and detector staging
+// touch memory; state is loaded at the start and written back at the end.
+// The arithmetic is the fused step's (yee.hip) operation for operation, so
+// results are bitwise those of the streaming kernels.
+//
+// Termination: every wait is bounded (spin_limit polls); a workgroup that
+// times out raises the error word and every waiting workgroup checks it, so
+// all waves exit.  The launch is cooperative, which guarantees co-residency
+// of the whole grid or fails.
+#include "common.hpp"
+
+namespace wfd {
+namespace {
+
+constexpr int kThreads = 1024;
+constexpr int kLdsFloats = 38912;           // 152 KB of state per workgroup
+constexpr int kResidentCells = kLdsFloats / 7;  // E (3) + H (3) + 1/eps (1) per cell
+
+struct ResArgs {
+  float* e;
+  float* h;
+  const float* inv_eps;
+  float* psi_e;
+  float* psi_h;
+  const float* coeff_e;
+  const float* coeff_h;
+  const float* src_e;
+  const float* src_h;
+  const float* wave_e;
+  const float* wave_h;
+  const int* counters;
+  int nstage, slots;
+  StageBox stages[WFD_MAX_STAGES];
+  int rows, steps, nx, ny, nz, n, source_x;
+  float courant;
+  int sx, sy, sz, gx, gy, gz;
+  float* face;
+  int* flags;  // [0, blocks): E flags, [blocks, 2 blocks): H flags
+  int* error;
+  int spin_limit;
+};
+
+// Make this workgroup's global stores visible at agent scope, then raise its
+// flag to v.
+__device__ __forceinline__ void publish(int* flag, int v) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(flag, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wait until the flags of up to three neighbours reach v (bounded).  Returns
+// false, uniformly across the workgroup, once the error word is raised.
+__device__ __forceinline__ bool wait_for(const ResArgs& a, const int* flags, int n0, int n1,
+                                         int n2, int v, int* bail) {
+  if (threadIdx.x == 0) {
+    int failed = 0;
+    const int nb[3] = {n0, n1, n2};
+    for (int k = 0; k < 3 && !failed; ++k) {
+      if (nb[k] < 0) continue;
+      int spins = 0;
+      while (__hip_atomic_load(flags + nb[k], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < v) {
+        if (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+            ++spins > a.spin_limit) {
+          __hip_atomic_store(a.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          failed = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    *bail = failed;
+  }
+  __syncthreads();
+  return *bail == 0;
+}
+
+__device__ __forceinline__ float cpml_step(float d, float a, float b, float k,
+                                           float* __restrict__ psi_at) {
+  if (a != 0.f) {
+    const float p = b * (*psi_at) + a * d;
+    *psi_at = p;
+    return k * d + p;
+  }
+  return k * d;
+}
+
+__global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
+  __shared__ float st[kLdsFloats];
+  __shared__ int bail;
+  const int tid = threadIdx.x;
+  const int wg = blockIdx.x;
+  const int blocks = gridDim.x;
+  const int bzi = wg % a.gz, byi = (wg / a.gz) % a.gy, bxi = wg / (a.gz * a.gy);
+  const int X0 = bxi * a.sx, Y0 = byi * a.sy, Z0 = bzi * a.sz;
+  const int vx = min(a.sx, a.nx - X0), vy = min(a.sy, a.ny - Y0), vz = min(a.sz, a.nz - Z0);
+  const int sy = a.sy, sz = a.sz;
+  const int cells = a.sx * sy * sz;  // LDS slots (the box may be partly outside)
+  float* E = st;                     // [3][cells]
+  float* H = st + 3 * cells;         // [3][cells]
+  float* IE = st + 6 * cells;        // [cells]
+  const int nx = a.nx, ny = a.ny, nz = a.nz, P = ny * nz, n = a.n;
+  const int sxy = a.gz * a.gy;
+  const int wxl = bxi > 0 ? wg - sxy : -1, wxh = bxi + 1 < a.gx ? wg + sxy : -1;
+  const int wyl = byi > 0 ? wg - a.gz : -1, wyh = byi + 1 < a.gy ? wg + a.gz : -1;
+  const int wzl = bzi > 0 ? wg - 1 : -1, wzh = bzi + 1 < a.gz ? wg + 1 : -1;
+  // face layout per workgroup: H high faces (x: 2*sy*sz, y: 2*sx*sz, z: 2*sx*sy),
+  // then E low faces of the same sizes.
+  const int fyz = sy * sz, fxz = a.sx * sz, fxy = a.sx * sy;
+  const size_t half = 2 * (size_t)(fyz + fxz + fxy);
+  const size_t per = 2 * half;
+  int* flagE = a.flags;
+  int* flagH = a.flags + blocks;
+  if (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+  const int t0 = a.counters[0];
+  const float S = a.courant;
+  const int rows = a.rows;
+  const float* __restrict__ ce = a.coeff_e;
+  const float* __restrict__ ch = a.coeff_h;
+
+  // load the sub-box (cells outside the grid hold zeros)
+  for (int c = tid; c < cells; c += kThreads) {
+    const int lz = c % sz, ly = (c / sz) % sy, lx = c / (sz * sy);
+    const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
+    float v[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (lx < vx && ly < vy && lz < vz) {
+      const int i = (x * ny + y) * nz + z;
+      v[0] = a.e[i];
+      v[1] = a.e[n + i];
+      v[2] = a.e[2 * n + i];
+      v[3] = a.h[i];
+      v[4] = a.h[n + i];
+      v[5] = a.h[2 * n + i];
+      v[6] = a.inv_eps[i];
+    }
+    for (int k = 0; k < 3; ++k) {
+      E[k * cells + c] = v[k];
+      H[k * cells + c] = v[3 + k];
+    }
+    IE[c] = v[6];
+  }
+  __syncthreads();
+
+  auto publish_h = [&](int v) {
+    float* f = a.face + (size_t)wg * per;
+    for (int c = tid; c < cells; c += kThreads) {
+      const int lz = c % sz, ly = (c / sz) % sy, lx = c / (sz * sy);
+      if (lx == vx - 1) {
+        f[ly * sz + lz] = H[cells + c];
+        f[fyz + ly * sz + lz] = H[2 * cells + c];
+      }
+      if (ly == vy - 1) {
+        f[2 * fyz + lx * sz + lz] = H[c];
+        f[2 * fyz + fxz + lx * sz + lz] = H[2 * cells + c];
+      }
+      if (lz == vz - 1) {
+        f[2 * fyz + 2 * fxz + lx * sy + ly] = H[c];
+        f[2 * fyz + 2 * fxz + fxy + lx * sy + ly] = H[cells + c];
+      }
+    }
+    publish(flagH + wg, v);
+  };
+  publish_h(1);
+
+  for (int k = 0; k < nsteps; ++k) {
+    const int t = t0 + k;
+    // ------------------------------------------------------------ E half
+    if (!wait_for(a, flagH, wxl, wyl, wzl, k + 1, &bail)) return;
+    {
+      const float* fx = wxl >= 0 ? a.face + (size_t)wxl * per : nullptr;
+      const float* fy = wyl >= 0 ? a.face + (size_t)wyl * per : nullptr;
+      const float* fz = wzl >= 0 ? a.face + (size_t)wzl * per : nullptr;
+      const bool src_here = a.source_x >= X0 && a.source_x < X0 + vx && t < a.steps;
+      const float wave_y = src_here ? a.wave_h[2 * a.steps + t] : 0.f;
+      const float wave_z = src_here ? a.wave_h[a.steps + t] : 0.f;
+      for (int c = tid; c < cells; c += kThreads) {
+        const int lz = c % sz, ly = (c / sz) % sy, lx = c / (sz * sy);
+        if (lx >= vx || ly >= vy || lz >= vz) continue;
+        const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
+        const int i = (x * ny + y) * nz + z;
+        const float hx = H[c], hy = H[cells + c], hz = H[2 * cells + c];
+        float my, mz, yx, yz, zx, zy;
+        if (lx > 0) {
+          my = H[cells + c - sy * sz];
+          mz = H[2 * cells + c - sy * sz];
+        } else if (fx) {
+          my = fx[ly * sz + lz];
+          mz = fx[fyz + ly * sz + lz];
+        } else {
+          my = mz = 0.f;
+        }
+        if (ly > 0) {
+          yx = H[c - sz];
+          yz = H[2 * cells + c - sz];
+        } else if (fy) {
+          yx = fy[2 * fyz + lx * sz + lz];
+          yz = fy[2 * fyz + fxz + lx * sz + lz];
+        } else {
+          yx = yz = 0.f;
+        }
+        if (lz > 0) {
+          zx = H[c - 1];
+          zy = H[cells + c - 1];
+        } else if (fz) {
+          zx = fz[2 * fyz + 2 * fxz + lx * sy + ly];
+          zy = fz[2 * fyz + 2 * fxz + fxy + lx * sy + ly];
+        } else {
+          zx = zy = 0.f;
+        }
+        const float dx1 = hy - my, dx2 = hz - mz;
+        const float dy0 = hx - yx, dy2 = hz - yz;
+        const float dz0 = hx - zx, dz1 = hy - zy;
+        const float* cx = ce + x * 3;
+        const float* cy = ce + (rows + y) * 3;
+        const float* cz = ce + (2 * rows + z) * 3;
+        float* psi = a.psi_e + i;
+        const float dxz = cpml_step(dx2, cx[0], cx[1], cx[2], psi);
+        const float dxy = cpml_step(dx1, cx[0], cx[1], cx[2], psi + n);
+        const float dyx = cpml_step(dy0, cy[0], cy[1], cy[2], psi + 2 * n);
+        const float dyz = cpml_step(dy2, cy[0], cy[1], cy[2], psi + 3 * n);
+        const float dzy = cpml_step(dz1, cz[0], cz[1], cz[2], psi + 4 * n);
+        const float dzx = cpml_step(dz0, cz[0], cz[1], cz[2], psi + 5 * n);
+        const float ie = IE[c];
+        float ex = E[c] + (S * (dyz - dzy)) * ie;
+        float ey = E[cells + c] + (S * (dzx - dxz)) * ie;
+        float ez = E[2 * cells + c] + (S * (dxy - dyx)) * ie;
+        if (src_here && x == a.source_x) {
+          const int o = y * nz + z;
+          ey += ((a.src_h[2 * P + o] * wave_y) * S) * ie;
+          ez -= ((a.src_h[P + o] * wave_z) * S) * ie;
+        }
+        E[c] = ex;
+        E[cells + c] = ey;
+        E[2 * cells + c] = ez;
+      }
+    }
+    __syncthreads();
+    {  // low E faces for the low-side neighbours' H half
+      float* f = a.face + (size_t)wg * per + half;
+      for (int c = tid; c < cells; c += kThreads) {
+        const int lz = c % sz, ly = (c / sz) % sy, lx = c / (sz * sy);
+        if (lx == 0) {
+          f[ly * sz + lz] = E[cells + c];
+          f[fyz + ly * sz + lz] = E[2 * cells + c];
+        }
+        if (ly == 0) {
+          f[2 * fyz + lx * sz + lz] = E[c];
+          f[2 * fyz + fxz + lx * sz + lz] = E[2 * cells + c];
+        }
+        if (lz == 0) {
+          f[2 * fyz + 2 * fxz + lx * sy + ly] = E[c];
+          f[2 * fyz + 2 * fxz + fxy + lx * sy + ly] = E[cells + c];
+        }
+      }
+      publish(flagE + wg, k + 1);
+    }
+    // ------------------------------------------------------------ H half
+    if (!wait_for(a, flagE, wxh, wyh, wzh, k + 1, &bail)) return;
+    {
+      const float* fx = wxh >= 0 ? a.face + (size_t)wxh * per + half : nullptr;
+      const float* fy = wyh >= 0 ? a.face + (size_t)wyh * per + half : nullptr;
+      const float* fz = wzh >= 0 ? a.face + (size_t)wzh * per + half : nullptr;
+      const bool src_here = a.source_x >= X0 && a.source_x < X0 + vx && t < a.steps;
+      const float wave_y = src_here ? a.wave_e[2 * a.steps + t] : 0.f;
+      const float wave_z = src_here ? a.wave_e[a.steps + t] : 0.f;
+      const int slot = a.nstage ? t % a.slots : 0;
+      for (int c = tid; c < cells; c += kThreads) {
+        const int lz = c % sz, ly = (c / sz) % sy, lx = c / (sz * sy);
+        if (lx >= vx || ly >= vy || lz >= vz) continue;
+        const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
+        const int i = (x * ny + y) * nz + z;
+        const float ex = E[c], ey = E[cells + c], ez = E[2 * cells + c];
+        float py, pz, qx, qz, rx, ry;
+        if (lx + 1 < vx) {
+          py = E[cells + c + sy * sz];
+          pz = E[2 * cells + c + sy * sz];
+        } else if (fx) {
+          py = fx[ly * sz + lz];
+          pz = fx[fyz + ly * sz + lz];
+        } else {
+          py = pz = 0.f;
+        }
+        if (ly + 1 < vy) {
+          qx = E[c + sz];
+          qz = E[2 * cells + c + sz];
+        } else if (fy) {
+          qx = fy[2 * fyz + lx * sz + lz];
+          qz = fy[2 * fyz + fxz + lx * sz + lz];
+        } else {
+          qx = qz = 0.f;
+        }
+        if (lz + 1 < vz) {
+          rx = E[c + 1];
+          ry = E[cells + c + 1];
+        } else if (fz) {
+          rx = fz[2 * fyz + 2 * fxz + lx * sy + ly];
+          ry = fz[2 * fyz + 2 * fxz + fxy + lx * sy + ly];
+        } else {
+          rx = ry = 0.f;
+        }
+        const float dx1 = py - ey, dx2 = pz - ez;
+        const float dy0 = qx - ex, dy2 = qz - ez;
+        const float dz0 = rx - ex, dz1 = ry - ey;
+        const float* cx = ch + x * 3;
+        const float* cy = ch + (rows + y) * 3;
+        const float* cz = ch + (2 * rows + z) * 3;
+        float* psi = a.psi_h + i;
+        const float dxz = cpml_step(dx2, cx[0], cx[1], cx[2], psi);
+        const float dxy = cpml_step(dx1, cx[0], cx[1], cx[2], psi + n);
+        const float dyx = cpml_step(dy0, cy[0], cy[1], cy[2], psi + 2 * n);
+        const float dyz = cpml_step(dy2, cy[0], cy[1], cy[2], psi + 3 * n);
+        const float dzy = cpml_step(dz1, cz[0], cz[1], cz[2], psi + 4 * n);
+        const float dzx = cpml_step(dz0, cz[0], cz[1], cz[2], psi + 5 * n);
+        const float ox = H[c], oy = H[cells + c], oz = H[2 * cells + c];
+        float hx = ox - S * (dyz - dzy);
+        float hy = oy - S * (dzx - dxz);
+        float hz = oz - S * (dxy - dyx);
+        if (src_here && x == a.source_x) {
+          const int o = y * nz + z;
+          hy -= (a.src_e[2 * P + o] * wave_y) * S;
+          hz += (a.src_e[P + o] * wave_z) * S;
+        }
+        H[c] = hx;
+        H[cells + c] = hy;
+        H[2 * cells + c] = hz;
+        for (int b = 0; b < a.nstage; ++b) {
+          const StageBox& s = a.stages[b];
+          if (x >= s.x0 && x < s.x1 && y >= s.y0 && y < s.y1 && z >= s.z0 && z < s.z1) {
+            const int cnt = s.count;
+            float* d = s.buf + (size_t)slot * 9 * cnt +
+                       ((x - s.x0) * (s.y1 - s.y0) + (y - s.y0)) * (s.z1 - s.z0) + (z - s.z0);
+            d[0] = ex;
+            d[cnt] = ey;
+            d[2 * cnt] = ez;
+            d[3 * cnt] = hx;
+            d[4 * cnt] = hy;
+            d[5 * cnt] = hz;
+            d[6 * cnt] = ox;
+            d[7 * cnt] = oy;
+            d[8 * cnt] = oz;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    publish_h(k + 2);
+  }
+  for (int c = tid; c < cells; c += kThreads) {
+    const int lz = c % sz, ly = (c / sz) % sy, lx = c / (sz * sy);
+    if (lx < vx && ly < vy && lz < vz) {
+      const int i = ((X0 + lx) * ny + Y0 + ly) * nz + Z0 + lz;
+      for (int k = 0; k < 3; ++k) {
+        a.e[k * n + i] = E[k * cells + c];
+        a.h[k * n + i] = H[k * cells + c];
+      }
+    }
+  }
+}
+
+__global__ void k_reset_flags(int* flags, int count) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+    flags[i] = 0;
+}
+
+__global__ void k_advance_counters(int* counters, int nsteps) {
+  counters[0] += nsteps;
+  counters[1] = counters[0] - 1;
+}
+
+struct Layout {
+  int sx, sy, sz, gx, gy, gz, blocks;
+};
+
+// Sub-box choice: the fewest cell passes per workgroup (ceil(cells / 1024)),
+// then the smaller sub-box surface (face traffic), then fewer workgroups.
+bool choose(int nx, int ny, int nz, int capacity, int want_sx, int want_sy, int want_sz,
+            Layout& best) {
+  bool found = false;
+  long best_key[3] = {0, 0, 0};
+  for (int sz = 1; sz <= nz; ++sz) {
+    if (want_sz && sz != want_sz) continue;
+    for (int sy = 1; sy <= ny; ++sy) {
+      if (want_sy && sy != want_sy) continue;
+      if (sy * sz > kResidentCells) break;
+      const int top = min(nx, kResidentCells / (sy * sz));
+      for (int sx = want_sx ? want_sx : 1; sx <= (want_sx ? want_sx : top); ++sx) {
+        if (sx * sy * sz > kResidentCells) break;
+        // even out the cells per workgroup along each axis without adding
+        // workgroups: an evened size never exceeds the first one
+        int gx = (nx + sx - 1) / sx, gy = (ny + sy - 1) / sy, gz = (nz + sz - 1) / sz;
+        const int ex = (nx + gx - 1) / gx, ey = (ny + gy - 1) / gy, ez = (nz + gz - 1) / gz;
+        gx = (nx + ex - 1) / ex;
+        gy = (ny + ey - 1) / ey;
+        gz = (nz + ez - 1) / ez;
+        const long blocks = (long)gx * gy * gz;
+        if (blocks > capacity) continue;
+        const long key[3] = {((long)ex * ey * ez + kThreads - 1) / kThreads,
+                             (long)ex * ey + (long)ex * ez + (long)ey * ez, blocks};
+        const bool better = !found || key[0] < best_key[0] ||
+                            (key[0] == best_key[0] &&
+                             (key[1] < best_key[1] ||
+                              (key[1] == best_key[1] && key[2] < best_key[2])));
+        if (better) {
+          found = true;
+          for (int q = 0; q < 3; ++q) best_key[q] = key[q];
+          best = Layout{ex, ey, ez, gx, gy, gz, (int)blocks};
+        }
+      }
+    }
+  }
+  return found;
+}
+
+}  // namespace
+}  // namespace wfd
+
+using namespace wfd;
+
+extern "C" {
+
+int wfd_resident_layout_for(const wfd_yee_plan* p, int sx, int sy, int sz,
+                            wfd_resident_layout* out) {
+  WFD_REQUIRE(p && out, "resident_layout: null argument");
+  WFD_REQUIRE(p->nx > 0 && p->ny > 0 && p->nz > 0 && cells(p->nx, p->ny, p->nz) <= kMaxCells,
+              "resident_layout: bad grid");
+  WFD_REQUIRE(sx >= 0 && sy >= 0 && sz >= 0, "resident_layout: negative sub-box");
+  int device = 0, cus = 0, per_cu = 0, coop = 0;
+  WFD_HIP(hipGetDevice(&device));
+  WFD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+  WFD_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device));
+  WFD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void*>(&k_resident), kThreads, 0));
+  const int capacity = per_cu * cus;
+  wfd_resident_layout L = {};
+  L.capacity = capacity;
+  if (!coop) {
+    *out = L;
+    set_error("resident_layout: device without cooperative launch");
+    return WFD_ERR_UNSUPPORTED;
+  }
+  Layout c;
+  if (!choose(p->nx, p->ny, p->nz, capacity, sx, sy, sz, c)) {
+    *out = L;
+    set_error("resident_layout: %dx%dx%d does not fit %d co-resident workgroups of %d cells",
+              p->nx, p->ny, p->nz, capacity, kResidentCells);
+    return WFD_ERR_UNSUPPORTED;
+  }
+  L.sx = c.sx;
+  L.sy = c.sy;
+  L.sz = c.sz;
+  L.gx = c.gx;
+  L.gy = c.gy;
+  L.gz = c.gz;
+  L.blocks = c.blocks;
+  const size_t per = 4 * (size_t)(c.sy * c.sz + c.sx * c.sz + c.sx * c.sy);
+  L.workspace_bytes =
+      (int64_t)(per * c.blocks * sizeof(float) + (2 * (size_t)c.blocks + 64) * sizeof(int));
+  *out = L;
+  return WFD_OK;
+}
+
+int wfd_resident_run(const wfd_yee_plan* p, const wfd_resident_layout* L, void* workspace,
+                     int nsteps, int spin_limit, void* stream) {
+  WFD_REQUIRE(p && L && workspace, "resident_run: null argument");
+  WFD_REQUIRE(nsteps >= 0, "resident_run: negative step count");
+  WFD_REQUIRE(p->e && p->h && p->inv_eps && p->psi_e && p->psi_h && p->coeff_e && p->coeff_h &&
+                  p->counters,
+              "resident_run: null field, psi, coeff or counter pointer");
+  WFD_REQUIRE(p->coeff_rows >= p->nx && p->coeff_rows >= p->ny && p->coeff_rows >= p->nz,
+              "resident_run: coeff_rows < max(shape)");
+  WFD_REQUIRE(p->source_x < p->nx, "resident_run: source plane outside the grid");
+  WFD_REQUIRE(p->source_x < 0 ||
+                  (p->src_e && p->src_h && p->wave_e && p->wave_h && p->steps > 0),
+              "resident_run: source plane without profiles/waves");
+  WFD_REQUIRE(p->nstage >= 0 && p->nstage <= WFD_MAX_STAGES, "resident_run: 0..%d stages",
+              WFD_MAX_STAGES);
+  WFD_REQUIRE(p->nstage == 0 || p->slots > 0, "resident_run: stage slots must be positive");
+  WFD_REQUIRE(L->sx > 0 && L->sy > 0 && L->sz > 0 && L->sx * L->sy * L->sz <= kResidentCells &&
+                  L->gx == (p->nx + L->sx - 1) / L->sx &&
+                  L->gy == (p->ny + L->sy - 1) / L->sy &&
+                  L->gz == (p->nz + L->sz - 1) / L->sz && L->blocks == L->gx * L->gy * L->gz &&
+                  L->blocks <= L->capacity,
+              "resident_run: layout does not match the plan");
+  const size_t per = 4 * (size_t)(L->sy * L->sz + L->sx * L->sz + L->sx * L->sy);
+  WFD_REQUIRE(L->workspace_bytes ==
+                  (int64_t)(per * L->blocks * sizeof(float) + (2 * (size_t)L->blocks + 64) * sizeof(int)),
+              "resident_run: workspace size does not match the layout");
+  if (nsteps == 0) return WFD_OK;
+  ResArgs a;
+  a.e = p->e;
+  a.h = p->h;
+  a.inv_eps = p->inv_eps;
+  a.psi_e = p->psi_e;
+  a.psi_h = p->psi_h;
+  a.coeff_e = p->coeff_e;
+  a.coeff_h = p->coeff_h;
+  a.src_e = p->src_e;
+  a.src_h = p->src_h;
+  a.wave_e = p->wave_e;
+  a.wave_h = p->wave_h;
+  a.counters = p->counters;
+  a.rows = p->coeff_rows;
+  a.steps = p->steps;
+  a.nx = p->nx;
+  a.ny = p->ny;
+  a.nz = p->nz;
+  a.n = p->nx * p->ny * p->nz;
+  a.source_x = p->source_x;
+  a.courant = p->courant;
+  a.sx = L->sx;
+  a.sy = L->sy;
+  a.sz = L->sz;
+  a.gx = L->gx;
+  a.gy = L->gy;
+  a.gz = L->gz;
+  a.face = reinterpret_cast<float*>(workspace);
+  a.flags = reinterpret_cast<int*>(a.face + per * L->blocks);
+  a.error = a.flags + 2 * L->blocks;
+  a.spin_limit = spin_limit > 0 ? spin_limit : (1 << 22);
+  a.nstage = p->nstage;
+  a.slots = p->slots > 0 ? p->slots : 1;
+  for (int b = 0; b < p->nstage; ++b)
+    WFD_REQUIRE(to_stage_box(p->stages[b], p->nx, p->ny, p->nz, a.stages[b]) == WFD_OK,
+                "resident_run: stage %d outside the grid or without a buffer", b);
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(k_reset_flags, dim3(8), dim3(256), 0, s, a.flags, 2 * L->blocks);
+  WFD_LAUNCH_CHECK("resident_run(reset flags)");
+  void* args[] = {&a, &nsteps};
+  WFD_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_resident),
+                                     dim3(L->blocks), dim3(kThreads), args, 0, s));
+  hipLaunchKernelGGL(k_advance_counters, dim3(1), dim3(1), 0, s, p->counters, nsteps);
+  WFD_LAUNCH_CHECK("resident_run(counters)");
+  return WFD_OK;
+}
+
+}  // extern "C"

@@ -4,7 +4,9 @@ Drop-in for warp_fdtd/solver.py: same constructor, validation errors,
 attributes and methods.  NumPy prepares the immutable tables on the host; the
 time loop runs as the fused E+H kernel pair (cpml_storage="full") or the
 per-cell packed-CPML kernels (cpml_storage="packed"), replayed through HIP
-graphs captured by this library.
+graphs captured by this library, or as the on-chip-resident cooperative kernel
+(wfd_resident_run) when the whole grid fits the LDS of the co-resident
+workgroups and every detector is staged.
 """
 
 from pathlib import Path
@@ -17,7 +19,7 @@ import numpy as np
 import torch
 
 from . import kernels as k
-from ._lib import ReplayDesc, WFD_MAX_STAGES, YeePlan, call, ptr, stream
+from ._lib import ReplayDesc, ResidentLayout, WFD_MAX_STAGES, YeePlan, call, ptr, stream
 from .array import Array
 
 
@@ -140,12 +142,20 @@ class Solver:
     read (their planes grown by the colocation stencil), which is all the
     detectors use.  VolumeMonitor or any other monitor type forces the full
     volume.
+
+    Extra keyword ``resident``: "auto" (default) lets ``compile`` time the
+    on-chip-resident stepping against the fused kernel pair and keep the
+    faster where it applies (full CPML storage, every monitor staged, no
+    hprev); True requires it (ValueError where it does not apply); False
+    never uses it.  Both give bitwise the same fields and detectors.
     """
 
     def __init__(self, case, device="cuda:0", graph_steps=64, cpml_storage="full",
-                 full_hprev=False, x_chunk=0):
+                 full_hprev=False, x_chunk=0, resident="auto"):
         if cpml_storage not in ("full", "packed"):
             raise ValueError("cpml_storage must be full or packed")
+        if resident not in ("auto", True, False):
+            raise ValueError('resident must be "auto", True or False')
         self.cpml_storage = cpml_storage
         self.path, self.meta, self.data = load_case(case)
         d, m = self.data, self.meta
@@ -234,6 +244,11 @@ class Solver:
         self._capture_stream = None
         self._flush_every = 1
         self._pending = 0
+        self.resident_mode = resident
+        self.resident = False
+        self.layout = None
+        self._workspace = None
+        self._compiled = False
 
     # ------------------------------------------------------------ stepping
     @property
@@ -329,7 +344,7 @@ class Solver:
         k.update_h_packed(self.H, self.E, self.eps, self.psi_h, self.coeffs["H"],
                           self.sources["E"], self.waves["E"], self.step, *common)
 
-    def _replay(self, m_steps):
+    def _replay_staged(self, m_steps):
         staged = self.staged()
         if not staged or m_steps == 0:
             return
@@ -351,12 +366,72 @@ class Solver:
         if staged:
             self._pending += 1
             if self._pending == self._flush_every:
-                self._replay(self._pending)
+                self._replay_staged(self._pending)
                 self._pending = 0
 
     def _finish_step(self):
         if self.cpml_storage != "full":
             k.advance(self.step)
+
+    # ------------------------------------------------- on-chip-resident path
+    def resident_applies(self):
+        """Whether the resident kernel can run this solver's step: full CPML
+        storage, every monitor recorded through footprint staging and no hprev
+        to save (it keeps no H_old outside the stage buffers)."""
+        if self.cpml_storage != "full" or self.full_hprev:
+            return False
+        if len(self.staged()) != len(self.monitors):
+            return False
+        box = self.hprev_box()
+        return box[1] <= box[0]
+
+    def resident_layout(self, sx=0, sy=0, sz=0):
+        """wfd_resident_layout_for: the sub-box decomposition (ValueError when
+        the grid does not fit the co-resident workgroups)."""
+        layout = ResidentLayout()
+        call("wfd_resident_layout_for", ctypes.byref(self._plan(stage=False)), int(sx), int(sy),
+             int(sz), ctypes.byref(layout))
+        return layout
+
+    def use_resident(self, layout=None):
+        """Switch stepping to the resident kernel with the given (default:
+        chosen) layout; allocates its zeroed workspace."""
+        if not self.resident_applies():
+            raise ValueError("resident stepping needs full CPML storage, staged monitors "
+                             "only and no hprev")
+        self.layout = layout if layout is not None else self.resident_layout()
+        words = int(self.layout.workspace_bytes) // 4
+        if self._workspace is None or self._workspace.shape[0] != words:
+            self._workspace = Array.zeros(words, np.int32, self.device)
+        self.resident = True
+
+    def _error_word(self):
+        L = self.layout
+        face = 4 * (L.sy * L.sz + L.sx * L.sz + L.sx * L.sy) * L.blocks
+        return face + 2 * L.blocks
+
+    def check(self):
+        """Raise if a resident run timed out waiting for a neighbouring
+        workgroup (its fields are then invalid); synchronizes."""
+        torch.cuda.synchronize(self.device)
+        if not self.resident:
+            return
+        i = self._error_word()
+        if int(self._workspace[i:i + 1].numpy()[0]) != 0:
+            self._workspace.zero_()
+            raise RuntimeError("resident stepping timed out waiting for a neighbouring "
+                               "workgroup; the fields of this run are invalid")
+
+    def resident_steps(self, count, record=True, spin_limit=0):
+        """`count` steps as one cooperative launch; staged monitors (record)
+        replay afterwards, so count must not exceed graph_steps then."""
+        if record and self.monitors and count > self.graph_steps:
+            raise ValueError("recorded resident chunks are at most graph_steps long")
+        plan = self._plan(stage=record)
+        call("wfd_resident_run", ctypes.byref(plan), ctypes.byref(self.layout),
+             ptr(self._workspace), int(count), int(spin_limit), stream())
+        if record:
+            self._replay_staged(count)
 
     def launch_step(self, record=True):
         """solver.py:194-237 — E, H, monitors, time advance."""
@@ -367,20 +442,23 @@ class Solver:
 
     def _capture_fn(self, length, fn):
         """HIP graph of `length` calls of fn, captured on a library stream;
-        staged monitors replay once at the end of the chunk."""
+        staged monitors replay every graph_steps steps and at the end."""
         if self._capture_stream is None:
             handle = ctypes.c_void_p()
             call("wfd_stream_create", ctypes.byref(handle))
             self._capture_stream = handle
         s = self._capture_stream
         torch.cuda.synchronize(self.device)
-        self._flush_every, self._pending = length, 0
+        # the stage rings hold graph_steps steps: longer graphs flush inside
+        self._flush_every, self._pending = min(length, self.graph_steps), 0
         call("wfd_graph_begin", s)
         ext = torch.cuda.ExternalStream(s.value, device=self.device)
         try:
             with torch.cuda.stream(ext):
                 for _ in range(length):
                     fn()
+                if self._pending:
+                    self._replay_staged(self._pending)
         finally:
             graph = ctypes.c_void_p()
             call("wfd_graph_end", s, ctypes.byref(graph))
@@ -391,11 +469,22 @@ class Solver:
         return self._capture_fn(length, lambda: self.launch_step(record=record))
 
     def tune(self):
-        """Pick the fused step's launch shape for this grid (autotune.py)."""
-        if self.cpml_storage == "full" and self.x_chunk == 0:
-            from .autotune import tune
+        """Pick the fused step's launch shape for this grid and, with
+        resident="auto", whether the resident kernel is faster (autotune.py)."""
+        from .autotune import tune, tune_resident
 
+        if self.cpml_storage == "full" and self.x_chunk == 0:
             self.x_chunk = tune(self)
+        self.resident = False
+        if self.resident_mode is True:
+            self.use_resident()
+        elif self.resident_mode == "auto" and self.resident_applies():
+            try:
+                layout = self.resident_layout()
+            except ValueError:
+                layout = None
+            if layout is not None and tune_resident(self, layout):
+                self.use_resident(layout)
         return self.x_chunk
 
     def compile(self):
@@ -406,11 +495,13 @@ class Solver:
         self.tune()
         self.reset()
         self._destroy_graphs()
-        for length in {self.graph_steps, self.steps % self.graph_steps} - {0}:
-            self.graphs[length] = self._capture(length, True)
-            if self.monitors:
-                self._quiet_graphs[length] = self._capture(length, False)
+        if not self.resident:
+            for length in {self.graph_steps, self.steps % self.graph_steps} - {0}:
+                self.graphs[length] = self._capture(length, True)
+                if self.monitors:
+                    self._quiet_graphs[length] = self._capture(length, False)
         self.reset()
+        self._compiled = True
         return time.perf_counter() - start
 
     def _windows(self):
@@ -420,15 +511,39 @@ class Solver:
             active |= m.active_steps(self.steps)
         return active
 
+    def _chunks(self):
+        return [self.graph_steps] * (self.steps // self.graph_steps) + (
+            [self.steps % self.graph_steps] if self.steps % self.graph_steps else [])
+
+    def _run_resident(self):
+        """Chunks with an active detector window run as recorded launches of
+        graph_steps; runs of quiet chunks merge into one launch."""
+        active = self._windows()
+        t, quiet = 0, 0
+        for length in self._chunks():
+            if self.monitors and active[t:t + length].any():
+                if quiet:
+                    self.resident_steps(quiet, record=False)
+                    quiet = 0
+                self.resident_steps(length, record=True)
+            else:
+                quiet += length
+            t += length
+        if quiet:
+            self.resident_steps(quiet, record=False)
+        self.check()
+
     def run(self):
         """solver.py:252-258 — replay the captured graphs over all steps."""
-        if not self.graphs:
+        if not self._compiled:
             raise RuntimeError("compile() must run before run()")
+        if self.resident:
+            self._run_resident()
+            return
         active = self._windows()
         s = stream()
         t = 0
-        for length in [self.graph_steps] * (self.steps // self.graph_steps) + (
-                [self.steps % self.graph_steps] if self.steps % self.graph_steps else []):
+        for length in self._chunks():
             quiet = self._quiet_graphs.get(length)
             graph = quiet if (quiet is not None and not active[t:t + length].any()) \
                 else self.graphs[length]
@@ -438,9 +553,19 @@ class Solver:
 
     def run_steps(self, count, synchronize=True):
         """Advance exactly `count` steps from the current step with graph
-        replays (graphs of missing lengths are captured on first use)."""
-        if not self.graphs:
+        replays (graphs of missing lengths are captured on first use), or
+        resident launches of at most graph_steps steps."""
+        if not self._compiled:
             raise RuntimeError("compile() must run before run_steps()")
+        if self.resident:
+            left = int(count)
+            while left > 0:
+                length = min(self.graph_steps, left) if self.monitors else left
+                self.resident_steps(length, record=bool(self.monitors))
+                left -= length
+            if synchronize:
+                self.check()
+            return
         s = stream()
         left = int(count)
         while left > 0:
