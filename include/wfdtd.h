@@ -300,6 +300,21 @@ int wfd_material_pullback(const float* weights, const float* filtered,
                           const float* grad_inv_eps, float* grad_p, float* scratch,
                           int sx, int sy, float beta, void* stream);
 
+/* ------------------------------------------------ linear density design map
+ * eps = eps_lo + rho * (eps_hi - eps_lo) through the sz z-layers of the design
+ * box (x0, y0, z0, sx, sy, sz); rho is (sx, sy) float64 with values in [0, 1].
+ * No reference counterpart: the reference's design map is material.py; this
+ * is the linear map of the density-parameterised bend (eps 2.085 .. 12.08).
+ * forward writes inv_eps = (float)(1 / eps) (float64 arithmetic, one rounding)
+ * into the box of the (nx, ny, nz) grid; pullback maps d/d(inv_eps) of the box
+ * cells, (sx, sy, sz) float64 as the adjoint accumulates it, to d/d(rho):
+ * grad_rho = sum_z grad_inv_eps * (-(eps_hi - eps_lo) / eps^2). */
+int wfd_density_forward(const double* rho, float* inv_eps, int nx, int ny, int nz, int x0,
+                        int y0, int z0, int sx, int sy, int sz, double eps_lo, double eps_hi,
+                        void* stream);
+int wfd_density_pullback(const double* rho, const double* grad_inv_eps, double* grad_rho, int sx,
+                         int sy, int sz, double eps_lo, double eps_hi, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,24 @@ def pack_coefficients(coeff, shape):
     return packed, max(1, *sizes)
 
 
+def pulse_table(meta, src, steps, offsets, half):
+    """Pulse sources of the synthetic workloads (no reference counterpart):
+    Gaussian exp(-u^2/2s^2) cos(w u + phi), u = t - delay, or Hann
+    sin^2(pi t/T) cos(w t + phi) on [0, T], evaluated in float64 on the
+    reference's time sampling t = (step + half + offset) dt and stored float32."""
+    t = (np.arange(steps, dtype=np.float64)[None, :] + half + offsets[:, :1]) * meta["dt_s"]
+    omega = 2 * np.pi * src["frequency_hz"]
+    phi = src["phase_shift"] + src["profile_phase_shift"]
+    if src["temporal_profile"] == "GaussianPulseProfile":
+        u = t - src["delay_s"]
+        wave = np.exp(-0.5 * (u / src["sigma_s"]) ** 2) * np.cos(omega * u + phi)
+    else:
+        inside = (t >= 0) & (t <= src["duration_s"])
+        wave = np.where(inside, np.sin(np.pi * t / src["duration_s"]) ** 2, 0.0) * np.cos(
+            omega * t + phi)
+    return (wave * src["static_amplitude_factor"]).astype(np.float32)
+
+
 def source_tables(meta, data, steps):
     """solver.py:132-182 — profiles (3, ny*nz) and waves (3, steps), float32."""
     src = next(o for o in meta["objects"] if o["class"] == "ModePlaneSource")
@@ -64,6 +82,9 @@ def source_tables(meta, data, steps):
     for field in "EH":
         profiles[field] = data[f"{name}/_{field}"].reshape(3, -1).astype(np.float32)
         offsets = data[f"{name}/_time_offset_{field}"].reshape(3, -1)
+        if src["temporal_profile"] in ("GaussianPulseProfile", "HannPulseProfile"):
+            waves[field] = pulse_table(meta, src, steps, offsets, 0.5 if field == "E" else 0.0)
+            continue
         half = np.float32(0.5 if field == "E" else 0.0)
         t = (np.arange(steps, dtype=np.float32)[None, :] + half + offsets[:, :1]) * np.float32(
             meta["dt_s"])
@@ -129,6 +150,43 @@ class Monitor:
         return float(abs(alpha) ** 2)
 
 
+class DFTPlane:
+    """Multi-wavelength running DFT plane (no reference counterpart; the
+    per-wavelength restatement of detectors.py:65-106's phasor branch):
+    phasor_f += colocated(E, H)(t) * exp(i 2 pi f t dt) dt, float64."""
+
+    def __init__(self, meta, data, steps, obj):
+        self.obj, self.name = obj, obj["name"]
+        self.shape = tuple(hi - lo for lo, hi in obj["grid_slice"])
+        self.lo = tuple(lo for lo, hi in obj["grid_slice"])
+        self.n = int(np.prod(self.shape))
+        self.axis = int(obj.get("propagation_axis", 0))
+        self.exact = bool(obj.get("exact_interpolation", True))
+        self.first = 0
+        self.wavelengths = np.asarray(obj["wavelengths_m"], np.float64)
+        f = 299792458.0 / self.wavelengths
+        t = np.arange(steps, dtype=np.float64) * float(meta["dt_s"])
+        w = np.exp(2j * np.pi * f[None, :] * t[:, None]) * float(meta["dt_s"])
+        self.weights = np.ascontiguousarray(np.stack((w.real, w.imag), axis=-1))
+        self.phasor = np.zeros((len(f), 6, self.n, 2))
+
+    def reset(self):
+        self.phasor[:] = 0
+
+    def phasors(self):
+        return (self.phasor[..., 0] + 1j * self.phasor[..., 1]).reshape(
+            len(self.wavelengths), 6, *self.shape)
+
+    def power(self, data, f):
+        """|alpha_f|^2, alpha = sum((E_m x H* + E* x H_m)[axis]) / 4."""
+        ph = self.phasors()[f]
+        em = data[self.name + "/_mode_E"][f]
+        hm = data[self.name + "/_mode_H"][f]
+        alpha = np.sum((np.cross(em, ph[3:].conj(), axis=0)
+                        + np.cross(ph[:3].conj(), hm, axis=0))[self.axis]) / 4
+        return float(abs(alpha) ** 2)
+
+
 class OracleSolver:
     """solver.py:53-264 restated on the CPU oracle kernels; ``dtype`` float32
     restates the reference, float64 gives the double-precision trajectory."""
@@ -170,7 +228,12 @@ class OracleSolver:
     def attach(self):
         """detectors.py:214-220"""
         for obj in self.meta["objects"]:
-            if obj["class"].endswith("Detector"):
+            if obj["class"] == "DFTPlaneDetector":
+                mon = DFTPlane(self.meta, self.data, self.steps, obj)
+                self.monitors.append(mon)
+                if mon.exact:
+                    self.save_h_from = 0
+            elif obj["class"].endswith("Detector"):
                 mon = Monitor(self.meta, self.data, self.steps, obj, self.dtype)
                 self.monitors.append(mon)
                 if mon.exact:
@@ -193,10 +256,27 @@ class OracleSolver:
                  self.sources["E"], self.waves["E"], self.step, self.hprev,
                  self.save_h_from, self.shape, self.source_x, self.courant, 0, self.packed)
         for mon in self.monitors:
+            if isinstance(mon, DFTPlane):
+                K.record_dft(self.E, self.H, self.hprev, int(self.step[0]), mon.weights,
+                             mon.phasor, self.shape, mon.lo, mon.shape, mon.exact)
+                continue
             K.record(self.E, self.H, self.hprev, self.step, mon.weights, mon.indices,
                      mon.phasor, mon.flux, mon.area, self.shape, mon.lo, mon.shape,
                      mon.axis, mon.exact, mon.is_phasor)
         self.step[0] += 1
+
+    def set_density(self, box, rho, eps_lo, eps_hi):
+        """Linear density map (no reference counterpart): inv_eps = 1/(eps_lo +
+        rho (eps_hi - eps_lo)) through the box's z-layers, float64 arithmetic."""
+        x0, y0, z0, sx, sy, sz = box
+        inv = 1 / (eps_lo + np.asarray(rho, np.float64) * (eps_hi - eps_lo))
+        eps = self.eps.reshape(self.shape)
+        eps[x0:x0 + sx, y0:y0 + sy, z0:z0 + sz] = inv.astype(self.dtype)[:, :, None]
+
+    def dft_transmission(self, f=0):
+        """P(mode_out) / P(mode_in) of the DFT planes at wavelength index f."""
+        planes = {m.name: m for m in self.monitors if isinstance(m, DFTPlane)}
+        return planes["mode_out"].power(self.data, f) / planes["mode_in"].power(self.data, f)
 
     def run(self, steps=None):
         for _ in range(self.steps if steps is None else steps):

@@ -132,6 +132,8 @@ SIGNATURES = {
     "wfd_record_mode": [P, P, P, P],
     "wfd_material_forward": [P, P, P, P, I, I, F, P],
     "wfd_material_pullback": [P, P, P, P, P, I, I, F, P],
+    "wfd_density_forward": [P, P] + [I] * 9 + [ctypes.c_double, ctypes.c_double, P],
+    "wfd_density_pullback": [P, P, P, I, I, I, ctypes.c_double, ctypes.c_double, P],
 }
 
 _lib = None

@@ -82,12 +82,19 @@ class Adjoint(Solver):
 
     ``box`` (x0, y0, z0, sx, sy, sz) and the design check default to the
     reference's 20 nm smooth bend; pass ``box`` explicitly (and
-    ``require_reference_bend=False``) for other bundles.
+    ``require_reference_bend=False``) for other bundles.  The objective is
+    the modal transmission T = P(mode_out) / P(mode_in) of the two modal
+    detectors, ModeOverlapDetector (the reference's) or multi-wavelength
+    DFTPlaneDetector planes, where ``objective_wavelength`` picks the
+    wavelength (optional when the planes hold one).
     """
 
     def __init__(self, case, checkpoint_every=None, grad_dtype=np.float64, box=None,
-                 require_reference_bend=True, **solver_kw):
+                 require_reference_bend=True, objective_wavelength=None, **solver_kw):
+        solver_kw.setdefault("resident", False)
         super().__init__(case, **solver_kw)
+        self.objective_wavelength = objective_wavelength
+        self._inject_weights = {}
         if require_reference_bend:
             if (self.shape != (200, 200, 75) or self.meta["settings"]["spacing_nm"] != 20
                     or self.meta["settings"]["parameterization"] != "smooth"):
@@ -137,7 +144,7 @@ class Adjoint(Solver):
             call("wfd_record_curl", ptr(self.H), ptr(self.history), int(self.history.shape[0]),
                  ptr(self.step), *self.shape, x0, y0, z0, sx, sy, sz, float(self.courant),
                  stream())
-        self._step_fields()
+        self._step_fields(stage=record)
         if record:
             self._record()
         self._finish_step()
@@ -146,6 +153,7 @@ class Adjoint(Solver):
         start = time.perf_counter()
         self.launch_step()
         torch.cuda.synchronize(self.device)
+        self.tune()
         self.reset()
         self._destroy_graphs()
         full = self.checkpoint_every is None
@@ -196,28 +204,72 @@ class Adjoint(Solver):
         torch.cuda.synchronize(self.device)
 
     # --------------------------------------------------------- reverse
+    def objective_index(self, m):
+        """Wavelength index of a DFT plane the objective reads."""
+        if self.objective_wavelength is None:
+            if m.nf != 1:
+                raise ValueError("objective_wavelength is required with multi-wavelength planes")
+            return 0
+        hit = np.flatnonzero(np.isclose(m.wavelengths, self.objective_wavelength, rtol=1e-9,
+                                        atol=0))
+        if not len(hit):
+            raise ValueError(f"{m.name} records no wavelength {self.objective_wavelength}")
+        return int(hit[0])
+
+    def _modal_terms(self, m):
+        """(phasor (6, *plane), mode E, mode H, area, inject weights) of the
+        objective's frequency."""
+        if hasattr(m, "nf"):  # DFT plane: alpha carries no area (detectors.DFTMonitor)
+            f = self.objective_index(m)
+            key = (m.name, f)
+            if key not in self._inject_weights:
+                w = m.weights.numpy()[:, f, :]
+                self._inject_weights[key] = Array.from_numpy(w, np.float32, self.device)
+            return (m.phasors()[f], self.data[m.name + "/_mode_E"][f],
+                    self.data[m.name + "/_mode_H"][f], np.ones(m.shape),
+                    self._inject_weights[key])
+        return (m.phasors(), self.data[m.name + "/_mode_E"][0], self.data[m.name + "/_mode_H"][0],
+                m.area_numpy, m.weights)
+
+    def transmission(self):
+        """T = P(mode_out) / P(mode_in) at the objective's wavelength."""
+        p = {}
+        for m in self.monitors:
+            if m.name in ("mode_in", "mode_out"):
+                ph, em, hm, area, _ = self._modal_terms(m)
+                beta = np.sum(self._overlap(m, em, hm, area) * ph)
+                p[m.name] = float(abs(beta) ** 2)
+        return p["mode_out"] / p["mode_in"]
+
+    @staticmethod
+    def _overlap(m, em, hm, area):
+        """coeff with sum(coeff * phasor) = conj(alpha) (adjoint.py:343-352)."""
+        a, b = (m.axis + 1) % 3, (m.axis + 2) % 3
+        qe = np.zeros_like(em)
+        qh = np.zeros_like(hm)
+        qe[a] = hm[b].conj() / 4
+        qe[b] = -hm[a].conj() / 4
+        qh[b] = em[a].conj() / 4
+        qh[a] = -em[b].conj() / 4
+        return np.concatenate((qe, qh)) * area[None]
+
     def prepare_sources(self):
         """adjoint.py:329-366 — host cotangents, refreshed in place."""
         previous = self.adjoint_sources
         monitors = {m.name: m for m in self.monitors}
-        pin = monitors["mode_in"].power(self)
-        pout = monitors["mode_out"].power(self)
+        terms = {name: self._modal_terms(monitors[name]) for name in ("mode_in", "mode_out")}
+        powers = {}
+        for name, (ph, em, hm, area, _) in terms.items():
+            beta = np.sum(self._overlap(monitors[name], em, hm, area) * ph)
+            powers[name] = float(abs(beta) ** 2)
+        pin, pout = powers["mode_in"], powers["mode_out"]
         if not np.isfinite(pin) or pin <= 0 or not np.isfinite(pout):
             raise ValueError("Finite modal powers and positive incident power required")
         self.adjoint_sources = []
         for name, factor in [("mode_in", pout / pin ** 2), ("mode_out", -1 / pin)]:
             m = monitors[name]
-            ph = m.phasors()
-            em = self.data[name + "/_mode_E"][0]
-            hm = self.data[name + "/_mode_H"][0]
-            a, b = (m.axis + 1) % 3, (m.axis + 2) % 3
-            qe = np.zeros_like(em)
-            qh = np.zeros_like(hm)
-            qe[a] = hm[b].conj() / 4
-            qe[b] = -hm[a].conj() / 4
-            qh[b] = em[a].conj() / 4
-            qh[a] = -em[b].conj() / 4
-            coeff = np.concatenate((qe, qh)) * m.area_numpy[None]
+            ph, em, hm, area, _ = terms[name]
+            coeff = self._overlap(m, em, hm, area)
             beta = np.sum(coeff * ph)
             source = (2 * factor * beta.conjugate() * coeff).reshape(6, -1).astype(np.complex64)
             host = np.stack((source.real, source.imag), axis=-1)
@@ -233,8 +285,14 @@ class Adjoint(Solver):
     def inject(self, old):
         for m, v in self.adjoint_sources:
             lo = [a for a, b in m.obj["grid_slice"]]
-            _inject(self.qe, self.qh, v, m.weights, self.reverse_step, *self.shape, *lo,
+            weights = self._inject_weights_of(m)
+            _inject(self.qe, self.qh, v, weights, self.reverse_step, *self.shape, *lo,
                     *m.shape[1:], old, dim=m.n)
+
+    def _inject_weights_of(self, m):
+        if hasattr(m, "nf"):
+            return self._inject_weights[(m.name, self.objective_index(m))]
+        return m.weights
 
     def reverse_one(self):
         """adjoint.py:388-414"""

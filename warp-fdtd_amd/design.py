@@ -1,0 +1,159 @@
+"""Density-parameterised inverse design on libwfdtd.
+
+The design is a density rho in [0, 1] on an (sx, sy) grid of in-plane voxels;
+the linear map eps = eps_lo + rho (eps_hi - eps_lo) fills the design box's
+z-layers (wfd_density_forward).  ``TransmissionProblem`` owns a checkpointed
+``Adjoint`` of a density bundle (cases.bend_case) and evaluates the modal
+transmission T at one wavelength and dT/drho: the adjoint accumulates
+d(-T)/d(inv_eps) over the box cells in float64, and wfd_density_pullback
+chains it through the map.  ``transmission(rho, problem)`` exposes the pair
+as a torch.autograd.Function.
+
+No reference counterpart: the reference's design path is the 80x80 filtered
+Material of adjoint.py:441-469 (kept in material.py / Adjoint.differentiate).
+"""
+
+import time
+
+import numpy as np
+import torch
+
+from ._lib import call, ptr, stream
+from .adjoint import Adjoint
+from .array import Array
+
+EPS_OXIDE = 2.085
+EPS_SI = 12.08
+
+
+class DensityMap:
+    """eps = eps_lo + rho (eps_hi - eps_lo) through the box (x0, y0, z0, sx, sy, sz)."""
+
+    def __init__(self, box, eps_lo=EPS_OXIDE, eps_hi=EPS_SI):
+        self.box = tuple(int(v) for v in box)
+        if len(self.box) != 6 or min(self.box[3:]) < 1:
+            raise ValueError("box is (x0, y0, z0, sx, sy, sz) with positive sizes")
+        if not (eps_lo > 0 and eps_hi > 0):
+            raise ValueError("Permittivities must be positive")
+        self.eps_lo, self.eps_hi = float(eps_lo), float(eps_hi)
+        self.plane = self.box[3:5]
+
+    def _rho(self, rho, device):
+        t = torch.as_tensor(rho, dtype=torch.float64, device=device)
+        if tuple(t.shape) != self.plane:
+            raise ValueError(f"rho must have shape {self.plane}")
+        if not torch.isfinite(t).all() or t.min() < 0 or t.max() > 1:
+            raise ValueError("rho must lie in [0, 1]")
+        return t.contiguous()
+
+    def write(self, rho, solver):
+        """Write 1/eps of the design into solver.eps (in place)."""
+        r = self._rho(rho, solver.device)
+        call("wfd_density_forward", ptr(r), ptr(solver.eps), *solver.shape, *self.box,
+             self.eps_lo, self.eps_hi, stream())
+        return r
+
+    def pullback(self, rho, grad_inv_eps):
+        """d/d(rho) of a scalar from its float64 d/d(inv_eps) over the box cells."""
+        g = getattr(grad_inv_eps, "tensor", grad_inv_eps)
+        if g.dtype != torch.float64 or g.numel() != int(np.prod(self.box[3:])):
+            raise ValueError("grad_inv_eps must be float64 over the design box")
+        r = self._rho(rho, g.device)
+        out = torch.empty(self.plane, dtype=torch.float64, device=g.device)
+        call("wfd_density_pullback", ptr(r), ptr(g.contiguous()), ptr(out), *self.box[3:],
+             self.eps_lo, self.eps_hi, stream())
+        return out
+
+
+class TransmissionProblem:
+    """Modal transmission T(rho) of a density bundle and its adjoint gradient.
+
+    ``case`` is a bundle written by cases.bend_case (its settings hold the
+    design box); ``wavelength`` picks the DFT wavelength of the objective
+    (optional when the planes hold one); ``checkpoint_every`` is the adjoint's
+    checkpoint interval (None keeps the full curl history).
+    """
+
+    def __init__(self, case, wavelength=None, checkpoint_every=64, device="cuda:0", **solver_kw):
+        from . import detectors
+
+        meta_box = _design_box(case)
+        self.adjoint = Adjoint(case, checkpoint_every=checkpoint_every, device=device,
+                               box=meta_box, require_reference_bend=False,
+                               objective_wavelength=wavelength, **solver_kw)
+        detectors.attach(self.adjoint)
+        self.map = DensityMap(meta_box)
+        self._forward_rho = None
+        self._compiled = False
+        self.timings = {}
+
+    def _compile(self):
+        if not self._compiled:
+            self.adjoint.compile()
+            self.adjoint.compile_reverse()
+            self._compiled = True
+
+    def forward(self, rho):
+        """T at the objective wavelength for the density rho (sx, sy)."""
+        self._compile()
+        a = self.adjoint
+        t0 = time.perf_counter()
+        r = self.map.write(rho, a)
+        a.reset()
+        a.run()
+        self._forward_rho = r.clone()
+        self.timings["forward_s"] = time.perf_counter() - t0
+        return a.transmission()
+
+    def backward(self, rho):
+        """dT/drho (sx, sy) float64 tensor for the design of the last forward
+        (re-runs the forward when rho differs from it)."""
+        r = self.map._rho(rho, self.adjoint.device)
+        if self._forward_rho is None or not torch.equal(r, self._forward_rho):
+            self.forward(r)
+        a = self.adjoint
+        t0 = time.perf_counter()
+        a.prepare_sources()
+        a.reset_reverse()
+        a.backward()
+        # the adjoint accumulates d(-T)/d(inv_eps)
+        out = -self.map.pullback(r, a.grad)
+        torch.cuda.synchronize(a.device)
+        self.timings["backward_s"] = time.perf_counter() - t0
+        return out
+
+    def value_and_grad(self, rho):
+        T = self.forward(rho)
+        return T, self.backward(rho)
+
+
+def _design_box(case):
+    from .solver import load_case
+
+    _, meta, _ = load_case(case)
+    box = meta["settings"].get("design_box")
+    if box is None:
+        raise ValueError("The bundle declares no design_box (write it with cases.bend_case)")
+    return tuple(int(v) for v in box)
+
+
+class Transmission(torch.autograd.Function):
+    """T(rho) with dT/drho from the adjoint."""
+
+    @staticmethod
+    def forward(ctx, rho, problem):
+        ctx.problem = problem
+        ctx.save_for_backward(rho)
+        value = problem.forward(rho.detach())
+        return torch.tensor(value, dtype=torch.float64, device=rho.device)
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        (rho,) = ctx.saved_tensors
+        grad = ctx.problem.backward(rho.detach())
+        return grad_output * grad.to(rho.dtype), None
+
+
+def transmission(rho, problem):
+    """Differentiable T(rho) (torch.autograd)."""
+    return Transmission.apply(rho, problem)
