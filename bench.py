@@ -123,6 +123,19 @@ def cpu_baseline(s, budget_s=12.0):
                       f"({elapsed:.1f} s), float32, OpenMP"}
 
 
+def copy_probe(mib):
+    """8 plain device copies of `mib` MiB: reads and writes mib MiB each, the
+    known traffic against which FETCH_SIZE / WRITE_SIZE are scaled."""
+    import torch
+
+    a = torch.ones(mib * 2 ** 18, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(8):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_mib": mib, "copies": 8}), flush=True)
+
+
 def main():
     p = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     p.add_argument("--gpus", type=int, default=1)
@@ -134,7 +147,13 @@ def main():
                    help="planes per thread column of the fused step (0: autotune)")
     p.add_argument("--resident", choices=("auto", "on", "off"), default="auto",
                    help="on-chip-resident stepping (auto: the faster of the two)")
+    p.add_argument("--copy-mib", type=int, default=0,
+                   help="only run 8 device-to-device copies of this many MiB (the reference "
+                        "traffic for calibrating HBM counters) and exit")
     args = p.parse_args()
+    if args.copy_mib:
+        copy_probe(args.copy_mib)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build "
               "targets a single MI355X; multi-GPU sharding is design-only)", file=sys.stderr)

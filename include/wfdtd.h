@@ -27,6 +27,7 @@ extern "C" {
 
 #define WFD_MAX_MONITORS 16
 #define WFD_MAX_STAGES 8
+#define WFD_MAX_BATCH 64
 
 /* ---------------------------------------------------------------- runtime */
 int wfd_version(void);
@@ -123,6 +124,22 @@ typedef struct wfd_yee_plan {
 } wfd_yee_plan;
 
 int wfd_yee_step(const wfd_yee_plan* plan, void* stream);
+
+/* ------------------------------------------------------------ batched grids
+ * Several independent grids (one wfd_yee_plan each, any shapes) stepped by
+ * one launch per half-step and x_chunk class: the launch concatenates every
+ * plan's block range and each block finds its plan in a table held in device
+ * memory.  Each plan's result is bitwise that of its own wfd_yee_step.  No
+ * reference counterpart (the reference steps one grid per Solver).
+ * prepare writes the table for `plans` into the workspace (workspace_bytes
+ * from wfd_batch_workspace_bytes, -1 for a bad count) and synchronizes the
+ * stream: call it outside graph capture, again whenever a plan changes.
+ * step must receive the same plans (pointers, shapes, x_chunk, stages) as the
+ * prepare that wrote the workspace. */
+int64_t wfd_batch_workspace_bytes(int count);
+int wfd_batch_prepare(const wfd_yee_plan* plans, int count, void* workspace,
+                      int64_t workspace_bytes, void* stream);
+int wfd_batch_step(const wfd_yee_plan* plans, int count, const void* workspace, void* stream);
 
 /* ------------------------------------------------- on-chip-resident stepping
  * The same step as wfd_yee_step (bitwise), for grids whose E, H and 1/eps fit

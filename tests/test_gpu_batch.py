@@ -1,0 +1,69 @@
+"""Batched stepping of uneven grids (wfd_batch_step) against single-grid runs."""
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 300
+
+
+@pytest.fixture(scope="module")
+def cases(tmp_path_factory, pkg):
+    root = tmp_path_factory.mktemp("batch")
+    c = pkg.cases
+    rho = np.random.default_rng(1).uniform(0, 1, (12, 12))
+    return [
+        c.bend_case(root / "bend", rho, shape=(48, 48, 24), spacing=40e-9, steps=STEPS, pml=6,
+                    design=(12, 12), pulse_duration=20e-15),
+        c.waveguide_case(root / "guide", shape=(64, 40, 32), steps=STEPS, pml=8),
+        c.slab_case(root / "slab", n=24, spacing=50e-9, steps=STEPS, pml=4, thickness=6),
+    ]
+
+
+def _solvers(pkg, cases, chunks):
+    out = []
+    for case, xc in zip(cases, chunks):
+        s = pkg.solver.Solver(case, x_chunk=xc, resident=False)
+        pkg.detectors.attach(s)
+        out.append(s)
+    return out
+
+
+@pytest.mark.parametrize("chunks", [(1, 2, 4), (2, 2, 8)])
+def test_batch_equals_single_runs(pkg, gpu, cases, chunks):
+    """Three shapes, mixed x-chunk classes, staged DFT planes: every design's
+    fields, CPML state, counters and phasors equal its own solo run bitwise."""
+    solo = _solvers(pkg, cases, chunks)
+    for s in solo:
+        s.compile()
+        s.reset()
+        s.run()
+    batch = pkg.batch.Batch(_solvers(pkg, cases, chunks))
+    batch.compile()
+    batch.run()
+    torch.cuda.synchronize()
+    for a, b in zip(solo, batch.solvers):
+        for name in ("E", "H", "psi_e", "psi_h", "counters"):
+            np.testing.assert_array_equal(getattr(a, name).numpy(), getattr(b, name).numpy(),
+                                          err_msg=name)
+        for ma, mb in zip(a.monitors, b.monitors):
+            np.testing.assert_array_equal(ma.phasor.numpy(), mb.phasor.numpy())
+    # replay after reset gives the same bits again
+    first = [s.E.numpy().copy() for s in batch.solvers]
+    batch.reset()
+    batch.run()
+    for f, s in zip(first, batch.solvers):
+        np.testing.assert_array_equal(f, s.E.numpy())
+
+
+def test_batch_validation(pkg, gpu, cases):
+    s = _solvers(pkg, cases, (2, 2, 2))
+    with pytest.raises(ValueError):
+        pkg.batch.Batch([])
+    packed = pkg.solver.Solver(cases[0], cpml_storage="packed")
+    with pytest.raises(ValueError):
+        pkg.batch.Batch([s[0], packed])
+    with pytest.raises(ValueError):
+        pkg.batch.Batch(s, graph_steps=32)

@@ -5,10 +5,10 @@ periodic, volume) over hand-written HIP kernels reached through the C ABI of
 libwfdtd.so (include/wfdtd.h).  Device arrays are torch tensors on ROCm.
 """
 
-from . import (_lib, adjoint, array, autotune, cases, convergence, design, detectors,
+from . import (_lib, adjoint, array, autotune, batch, cases, convergence, design, detectors,
                driven_uniform, guide_observer, guide_problem, guide_validation, kernels,
                material, modes, periodic, solver, volume)
 
-__all__ = ["_lib", "adjoint", "array", "autotune", "cases", "convergence", "design",
+__all__ = ["_lib", "adjoint", "array", "autotune", "batch", "cases", "convergence", "design",
            "detectors", "driven_uniform", "guide_observer", "guide_problem", "guide_validation",
            "kernels", "material", "modes", "periodic", "solver", "volume"]

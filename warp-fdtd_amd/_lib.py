@@ -26,6 +26,7 @@ F = ctypes.c_float
 
 
 WFD_MAX_STAGES = 8
+WFD_MAX_BATCH = 64
 
 
 class StageDesc(ctypes.Structure):
@@ -111,6 +112,9 @@ SIGNATURES = {
     "wfd_update_h_packed": _UPDATE,
     "wfd_advance": [P, I, P],
     "wfd_yee_step": [ctypes.POINTER(YeePlan), P],
+    "wfd_batch_workspace_bytes": [I],
+    "wfd_batch_prepare": [ctypes.POINTER(YeePlan), I, P, L, P],
+    "wfd_batch_step": [ctypes.POINTER(YeePlan), I, P, P],
     "wfd_resident_layout_for": [ctypes.POINTER(YeePlan), I, I, I, ctypes.POINTER(ResidentLayout)],
     "wfd_resident_run": [ctypes.POINTER(YeePlan), ctypes.POINTER(ResidentLayout), P, I, I, P],
     "wfd_record": [P, P, P, P, P, P, P, P, P] + [I] * 13 + [P],
@@ -150,7 +154,8 @@ def lib():
         for name, args in SIGNATURES.items():
             fn = getattr(handle, name)
             fn.argtypes = args
-            fn.restype = ctypes.c_char_p if name == "wfd_last_error" else ctypes.c_int
+            fn.restype = {"wfd_last_error": ctypes.c_char_p,
+                          "wfd_batch_workspace_bytes": ctypes.c_int64}.get(name, ctypes.c_int)
         _lib = handle
     return _lib
 

@@ -1,0 +1,144 @@
+"""Many independent grids stepped together (no reference counterpart).
+
+``Batch`` holds one ``Solver`` per design (any shapes, the same number of
+steps) and advances all of them with ``wfd_batch_step``: one launch per
+half-step and x-chunk class covers every design's blocks, each design keeps
+its own fields, CPML, source, counters and staged detectors, and its results
+are bitwise those of stepping it alone.  ``compile`` tunes every design's x
+chunk (autotune.py, cached by shape), writes the device argument tables and
+captures one HIP graph per chunk of ``graph_steps`` steps that steps every
+design and replays every design's staged detectors; ``run`` replays them.
+"""
+
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from ._lib import WFD_MAX_BATCH, YeePlan, call, ptr, stream
+from .array import Array
+
+
+class Batch:
+    def __init__(self, solvers, graph_steps=64):
+        self.solvers = list(solvers)
+        if not 1 <= len(self.solvers) <= WFD_MAX_BATCH:
+            raise ValueError(f"A batch holds 1..{WFD_MAX_BATCH} solvers")
+        steps = {s.steps for s in self.solvers}
+        if len(steps) != 1:
+            raise ValueError("Every design of a batch must run the same number of steps")
+        if any(s.cpml_storage != "full" for s in self.solvers):
+            raise ValueError("Batched stepping needs the full CPML storage")
+        if any(len(s.staged()) != len(s.monitors) for s in self.solvers):
+            raise ValueError("Batched stepping records staged detectors only")
+        if graph_steps < 1 or any(s.graph_steps != graph_steps for s in self.solvers):
+            raise ValueError("graph_steps must be positive and equal to every solver's")
+        self.steps = steps.pop()
+        self.graph_steps = graph_steps
+        self.device = self.solvers[0].device
+        self.graphs, self.quiet_graphs = {}, {}
+        self._stream = None
+        self._tables = {}
+        self.cells = sum(s.n for s in self.solvers)
+
+    # ------------------------------------------------------------ tables
+    def _table(self, record):
+        plans = (YeePlan * len(self.solvers))(*[s._plan(stage=record) for s in self.solvers])
+        nbytes = int(call_bytes(len(self.solvers)))
+        ws = Array.zeros((nbytes + 3) // 4, np.int32, self.device)
+        call("wfd_batch_prepare", plans, len(self.solvers), ptr(ws), nbytes, stream())
+        return plans, ws
+
+    def step(self, record=True):
+        """One step of every design (eager); staged detectors replay each step."""
+        plans, ws = self._tables[record]
+        call("wfd_batch_step", plans, len(self.solvers), ptr(ws), stream())
+        if record:
+            for s in self.solvers:
+                s._replay_staged(1)
+
+    def _capture(self, length, record):
+        if self._stream is None:
+            handle = ctypes.c_void_p()
+            call("wfd_stream_create", ctypes.byref(handle))
+            self._stream = handle
+        s = self._stream
+        plans, ws = self._tables[record]
+        torch.cuda.synchronize(self.device)
+        call("wfd_graph_begin", s)
+        ext = torch.cuda.ExternalStream(s.value, device=self.device)
+        try:
+            with torch.cuda.stream(ext):
+                done = 0
+                while done < length:
+                    m = min(self.graph_steps, length - done)
+                    for _ in range(m):
+                        call("wfd_batch_step", plans, len(self.solvers), ptr(ws), stream())
+                    if record:
+                        for solver in self.solvers:
+                            solver._replay_staged(m)
+                    done += m
+        finally:
+            graph = ctypes.c_void_p()
+            call("wfd_graph_end", s, ctypes.byref(graph))
+        return graph
+
+    # ------------------------------------------------------------ driver
+    def compile(self):
+        start = time.perf_counter()
+        for s in self.solvers:
+            s.resident_mode = False
+            s.launch_step()
+            torch.cuda.synchronize(self.device)
+            s.tune()
+            s.reset()
+        self._destroy_graphs()
+        self._tables = {True: self._table(True), False: self._table(False)}
+        for length in {self.graph_steps, self.steps % self.graph_steps} - {0}:
+            self.graphs[length] = self._capture(length, True)
+            self.quiet_graphs[length] = self._capture(length, False)
+        self.reset()
+        return time.perf_counter() - start
+
+    def reset(self):
+        for s in self.solvers:
+            s.reset()
+
+    def run(self):
+        """Every design's full trajectory: one graph launch per chunk."""
+        if not self.graphs:
+            raise RuntimeError("compile() must run before run()")
+        active = np.zeros(self.steps, bool)
+        for s in self.solvers:
+            active |= s._windows()
+        t = 0
+        for length in [self.graph_steps] * (self.steps // self.graph_steps) + (
+                [self.steps % self.graph_steps] if self.steps % self.graph_steps else []):
+            g = self.graphs[length] if active[t:t + length].any() else self.quiet_graphs[length]
+            call("wfd_graph_launch", g, stream())
+            t += length
+        torch.cuda.synchronize(self.device)
+
+    def _destroy_graphs(self):
+        for table in (self.graphs, self.quiet_graphs):
+            for g in table.values():
+                call("wfd_graph_destroy", g)
+            table.clear()
+
+    def __del__(self):
+        try:
+            self._destroy_graphs()
+            if self._stream is not None:
+                call("wfd_stream_destroy", self._stream)
+        except Exception:
+            pass
+
+
+def call_bytes(count):
+    from ._lib import lib
+
+    nbytes = lib().wfd_batch_workspace_bytes(int(count))
+    if nbytes < 0:
+        raise ValueError(f"A batch holds 1..{WFD_MAX_BATCH} plans")
+    return nbytes

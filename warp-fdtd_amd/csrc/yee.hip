@@ -179,8 +179,8 @@ struct Cell {
 };
 
 // Decode the XCD-striped block id; false for padding blocks and idle lanes.
-__device__ __forceinline__ bool locate(const YeeArgs& a, int xc, int& logical, Cell& c) {
-  const int bid = blockIdx.x;
+__device__ __forceinline__ bool locate(const YeeArgs& a, int bid, int xc, int& logical,
+                                       Cell& c) {
   logical = (bid & 7) * a.per + (bid >> 3);
   if (logical >= a.total) return false;
   const int chunk = logical / a.segs;
@@ -194,10 +194,10 @@ __device__ __forceinline__ bool locate(const YeeArgs& a, int xc, int& logical, C
 }
 
 template <int XC>
-__global__ __launch_bounds__(kBlock) void k_yee_e(YeeArgs a) {
+__device__ __forceinline__ void yee_e(const YeeArgs& a, int bid) {
   int logical;
   Cell c;
-  if (!locate(a, XC, logical, c)) return;
+  if (!locate(a, bid, XC, logical, c)) return;
   const int t = a.counters[0];
   if (logical == 0 && threadIdx.x == 0) a.counters[1] = t;
   if (c.o >= a.P) return;
@@ -312,10 +312,10 @@ __global__ __launch_bounds__(kBlock) void k_yee_e(YeeArgs a) {
 }
 
 template <int XC>
-__global__ __launch_bounds__(kBlock) void k_yee_h(YeeArgs a) {
+__device__ __forceinline__ void yee_h(const YeeArgs& a, int bid) {
   int logical;
   Cell c;
-  if (!locate(a, XC, logical, c)) return;
+  if (!locate(a, bid, XC, logical, c)) return;
   const int t = a.counters[1];
   if (logical == 0 && threadIdx.x == 0) a.counters[0] = t + 1;
   if (c.o >= a.P) return;
@@ -441,6 +441,115 @@ __global__ __launch_bounds__(kBlock) void k_yee_h(YeeArgs a) {
   }
 }
 
+template <int XC>
+__global__ __launch_bounds__(kBlock) void k_yee_e(YeeArgs a) {
+  yee_e<XC>(a, blockIdx.x);
+}
+
+template <int XC>
+__global__ __launch_bounds__(kBlock) void k_yee_h(YeeArgs a) {
+  yee_h<XC>(a, blockIdx.x);
+}
+
+// Batched step: the grid concatenates the (XCD-striped, padded) block ranges
+// of `count` plans of one x-chunk class; first[d] is the first block of plan
+// d (first[count] = the grid size).  The per-cell arithmetic is the single
+// plan's, so every plan's result is bitwise its own wfd_yee_step's.
+__device__ __forceinline__ int batch_member(const int* __restrict__ first, int count, int bid) {
+  int d = 0;
+  while (d + 1 < count && first[d + 1] <= bid) ++d;
+  return d;
+}
+
+template <int XC>
+__global__ __launch_bounds__(kBlock) void k_batch_e(const YeeArgs* __restrict__ table,
+                                                    const int* __restrict__ first, int count) {
+  const int d = batch_member(first, count, blockIdx.x);
+  yee_e<XC>(table[d], blockIdx.x - first[d]);
+}
+
+template <int XC>
+__global__ __launch_bounds__(kBlock) void k_batch_h(const YeeArgs* __restrict__ table,
+                                                    const int* __restrict__ first, int count) {
+  const int d = batch_member(first, count, blockIdx.x);
+  yee_h<XC>(table[d], blockIdx.x - first[d]);
+}
+
+// Validate a plan and fill the fused step's arguments (shared by the single
+// and the batched step).
+int build_args(const wfd_yee_plan* p, YeeArgs& a, const char* who) {
+  WFD_REQUIRE(p, "%s: null plan", who);
+  WFD_REQUIRE(p->nx > 0 && p->ny > 0 && p->nz > 0, "%s: grid dimensions must be positive", who);
+  WFD_REQUIRE(cells(p->nx, p->ny, p->nz) <= kMaxCells, "%s: grid exceeds 2^31 cells", who);
+  WFD_REQUIRE((int64_t)p->ny * p->nz < (1 << 24), "%s: ny*nz must be < 2^24", who);
+  WFD_REQUIRE(p->e && p->h && p->inv_eps && p->psi_e && p->psi_h && p->coeff_e &&
+                  p->coeff_h && p->counters,
+              "%s: null field, psi, coeff or counter pointer", who);
+  WFD_REQUIRE(p->coeff_rows >= p->nx && p->coeff_rows >= p->ny && p->coeff_rows >= p->nz,
+              "%s: coeff_rows < max(shape)", who);
+  WFD_REQUIRE(p->source_x < p->nx, "%s: source plane outside the grid", who);
+  WFD_REQUIRE(p->source_x < 0 ||
+                  (p->src_e && p->src_h && p->wave_e && p->wave_h && p->steps > 0),
+              "%s: source plane without profiles/waves", who);
+  WFD_REQUIRE(p->save_h_from >= p->steps || p->hprev, "%s: hprev required", who);
+  a.e = p->e;
+  a.h = p->h;
+  a.inv_eps = p->inv_eps;
+  a.psi_e = p->psi_e;
+  a.psi_h = p->psi_h;
+  a.coeff_e = p->coeff_e;
+  a.coeff_h = p->coeff_h;
+  a.src_e = p->src_e;
+  a.src_h = p->src_h;
+  a.wave_e = p->wave_e;
+  a.wave_h = p->wave_h;
+  a.counters = p->counters;
+  a.hprev = p->hprev;
+  for (int k = 0; k < 6; ++k) a.hbox[k] = p->hbox[k];
+  a.save_h_from = p->save_h_from;
+  a.rows = p->coeff_rows;
+  a.steps = p->steps;
+  a.nx = p->nx;
+  a.ny = p->ny;
+  a.nz = p->nz;
+  a.n = p->nx * p->ny * p->nz;
+  a.P = p->ny * p->nz;
+  a.inv_nz = 1.0f / (float)p->nz;
+  a.source_x = p->source_x;
+  a.courant = p->courant;
+  a.x_chunk = p->x_chunk > 0 ? p->x_chunk : 2;
+  WFD_REQUIRE(a.x_chunk == 1 || a.x_chunk == 2 || a.x_chunk == 4 || a.x_chunk == 8,
+              "%s: x_chunk must be 1, 2, 4 or 8", who);
+  WFD_REQUIRE(p->nstage >= 0 && p->nstage <= WFD_MAX_STAGES, "%s: 0..%d stages", who,
+              WFD_MAX_STAGES);
+  WFD_REQUIRE(p->nstage == 0 || p->slots > 0, "%s: stage slots must be positive", who);
+  a.nstage = p->nstage;
+  a.slots = p->slots > 0 ? p->slots : 1;
+  for (int b = 0; b < WFD_MAX_STAGES; ++b) a.st[b] = StageBox{};
+  for (int b = 0; b < p->nstage; ++b)
+    WFD_REQUIRE(to_stage_box(p->stages[b], p->nx, p->ny, p->nz, a.st[b]) == WFD_OK,
+                "%s: stage %d outside the grid or without a buffer", who, b);
+  a.segs = (a.P + kBlock - 1) / kBlock;
+  const int chunks = (a.nx + a.x_chunk - 1) / a.x_chunk;
+  a.total = a.segs * chunks;
+  a.per = (a.total + 7) / 8;
+  return WFD_OK;
+}
+
+constexpr int kClasses[4] = {1, 2, 4, 8};
+
+// Batch table order: plans grouped by x-chunk class (1, 2, 4, 8), each group in
+// the caller's order.  order[k] = plan index of table entry k.
+void batch_order(const YeeArgs* args, int count, int* order, int* class_start) {
+  int k = 0;
+  for (int c = 0; c < 4; ++c) {
+    class_start[c] = k;
+    for (int d = 0; d < count; ++d)
+      if (args[d].x_chunk == kClasses[c]) order[k++] = d;
+  }
+  class_start[4] = k;
+}
+
 template <bool ELECTRIC, bool PACKED>
 int launch_update(float* f, const float* g, const float* inv_eps, float* psi,
                   int64_t psi_stride, const float* coeff, int rows, const float* source,
@@ -497,61 +606,8 @@ int wfd_advance(int* step, int delta, void* stream) {
 }
 
 int wfd_yee_step(const wfd_yee_plan* p, void* stream) {
-  WFD_REQUIRE(p, "yee_step: null plan");
-  WFD_REQUIRE(p->nx > 0 && p->ny > 0 && p->nz > 0, "yee_step: grid dimensions must be positive");
-  WFD_REQUIRE(cells(p->nx, p->ny, p->nz) <= kMaxCells, "yee_step: grid exceeds 2^31 cells");
-  WFD_REQUIRE((int64_t)p->ny * p->nz < (1 << 24), "yee_step: ny*nz must be < 2^24");
-  WFD_REQUIRE(p->e && p->h && p->inv_eps && p->psi_e && p->psi_h && p->coeff_e &&
-                  p->coeff_h && p->counters,
-              "yee_step: null field, psi, coeff or counter pointer");
-  WFD_REQUIRE(p->coeff_rows >= p->nx && p->coeff_rows >= p->ny && p->coeff_rows >= p->nz,
-              "yee_step: coeff_rows < max(shape)");
-  WFD_REQUIRE(p->source_x < p->nx, "yee_step: source plane outside the grid");
-  WFD_REQUIRE(p->source_x < 0 ||
-                  (p->src_e && p->src_h && p->wave_e && p->wave_h && p->steps > 0),
-              "yee_step: source plane without profiles/waves");
-  WFD_REQUIRE(p->save_h_from >= p->steps || p->hprev, "yee_step: hprev required");
   YeeArgs a;
-  a.e = p->e;
-  a.h = p->h;
-  a.inv_eps = p->inv_eps;
-  a.psi_e = p->psi_e;
-  a.psi_h = p->psi_h;
-  a.coeff_e = p->coeff_e;
-  a.coeff_h = p->coeff_h;
-  a.src_e = p->src_e;
-  a.src_h = p->src_h;
-  a.wave_e = p->wave_e;
-  a.wave_h = p->wave_h;
-  a.counters = p->counters;
-  a.hprev = p->hprev;
-  for (int k = 0; k < 6; ++k) a.hbox[k] = p->hbox[k];
-  a.save_h_from = p->save_h_from;
-  a.rows = p->coeff_rows;
-  a.steps = p->steps;
-  a.nx = p->nx;
-  a.ny = p->ny;
-  a.nz = p->nz;
-  a.n = p->nx * p->ny * p->nz;
-  a.P = p->ny * p->nz;
-  a.inv_nz = 1.0f / (float)p->nz;
-  a.source_x = p->source_x;
-  a.courant = p->courant;
-  a.x_chunk = p->x_chunk > 0 ? p->x_chunk : 2;
-  WFD_REQUIRE(a.x_chunk == 1 || a.x_chunk == 2 || a.x_chunk == 4 || a.x_chunk == 8,
-              "yee_step: x_chunk must be 1, 2, 4 or 8");
-  WFD_REQUIRE(p->nstage >= 0 && p->nstage <= WFD_MAX_STAGES, "yee_step: 0..%d stages",
-              WFD_MAX_STAGES);
-  WFD_REQUIRE(p->nstage == 0 || p->slots > 0, "yee_step: stage slots must be positive");
-  a.nstage = p->nstage;
-  a.slots = p->slots > 0 ? p->slots : 1;
-  for (int b = 0; b < p->nstage; ++b)
-    WFD_REQUIRE(to_stage_box(p->stages[b], p->nx, p->ny, p->nz, a.st[b]) == WFD_OK,
-                "yee_step: stage %d outside the grid or without a buffer", b);
-  a.segs = (a.P + kBlock - 1) / kBlock;
-  const int chunks = (a.nx + a.x_chunk - 1) / a.x_chunk;
-  a.total = a.segs * chunks;
-  a.per = (a.total + 7) / 8;
+  if (int status = build_args(p, a, "yee_step")) return status;
   const dim3 grid(a.per * 8), block(kBlock);
   hipStream_t s = as_stream(stream);
   switch (a.x_chunk) {
@@ -573,6 +629,85 @@ int wfd_yee_step(const wfd_yee_plan* p, void* stream) {
       break;
   }
   WFD_LAUNCH_CHECK("yee_step");
+  return WFD_OK;
+}
+
+int64_t wfd_batch_workspace_bytes(int count) {
+  if (count < 1 || count > WFD_MAX_BATCH) return -1;
+  return (int64_t)count * sizeof(YeeArgs) + (int64_t)(count + 4) * sizeof(int);
+}
+
+int wfd_batch_prepare(const wfd_yee_plan* plans, int count, void* workspace,
+                      int64_t workspace_bytes, void* stream) {
+  WFD_REQUIRE(plans && workspace, "batch_prepare: null argument");
+  WFD_REQUIRE(count >= 1 && count <= WFD_MAX_BATCH, "batch_prepare: 1..%d plans",
+              WFD_MAX_BATCH);
+  WFD_REQUIRE(workspace_bytes >= wfd_batch_workspace_bytes(count),
+              "batch_prepare: workspace smaller than wfd_batch_workspace_bytes");
+  YeeArgs args[WFD_MAX_BATCH];
+  for (int d = 0; d < count; ++d)
+    if (int status = build_args(plans + d, args[d], "batch_prepare")) return status;
+  int order[WFD_MAX_BATCH], start[5];
+  batch_order(args, count, order, start);
+  YeeArgs table[WFD_MAX_BATCH];
+  int first[WFD_MAX_BATCH + 4];
+  int f = 0;
+  for (int c = 0; c < 4; ++c) {
+    int at = 0;
+    for (int k = start[c]; k < start[c + 1]; ++k) {
+      table[k] = args[order[k]];
+      first[f++] = at;
+      at += table[k].per * 8;
+    }
+    first[f++] = at;  // class end
+  }
+  hipStream_t s = as_stream(stream);
+  char* ws = static_cast<char*>(workspace);
+  WFD_HIP(hipMemcpyAsync(ws, table, sizeof(YeeArgs) * count, hipMemcpyHostToDevice, s));
+  WFD_HIP(hipMemcpyAsync(ws + sizeof(YeeArgs) * count, first, sizeof(int) * (count + 4),
+                         hipMemcpyHostToDevice, s));
+  WFD_HIP(hipStreamSynchronize(s));
+  return WFD_OK;
+}
+
+int wfd_batch_step(const wfd_yee_plan* plans, int count, const void* workspace, void* stream) {
+  WFD_REQUIRE(plans && workspace, "batch_step: null argument");
+  WFD_REQUIRE(count >= 1 && count <= WFD_MAX_BATCH, "batch_step: 1..%d plans", WFD_MAX_BATCH);
+  YeeArgs args[WFD_MAX_BATCH];
+  for (int d = 0; d < count; ++d)
+    if (int status = build_args(plans + d, args[d], "batch_step")) return status;
+  int order[WFD_MAX_BATCH], start[5];
+  batch_order(args, count, order, start);
+  const YeeArgs* table = static_cast<const YeeArgs*>(workspace);
+  const int* first = reinterpret_cast<const int*>(table + count);
+  hipStream_t s = as_stream(stream);
+  const dim3 block(kBlock);
+  // E half-steps of every class, then H half-steps (H of a plan needs only
+  // that plan's E, so the class order is free)
+  for (int half = 0; half < 2; ++half) {
+    for (int c = 0; c < 4; ++c) {
+      const int members = start[c + 1] - start[c];
+      if (!members) continue;
+      int blocks = 0;
+      for (int k = start[c]; k < start[c + 1]; ++k) blocks += args[order[k]].per * 8;
+      const YeeArgs* t = table + start[c];
+      const int* fc = first + start[c] + c;
+      const dim3 grid(blocks);
+#define WFD_BATCH_LAUNCH(XC)                                                          \
+  if (half == 0)                                                                      \
+    hipLaunchKernelGGL(k_batch_e<XC>, grid, block, 0, s, t, fc, members);             \
+  else                                                                                \
+    hipLaunchKernelGGL(k_batch_h<XC>, grid, block, 0, s, t, fc, members);
+      switch (kClasses[c]) {
+        case 1: WFD_BATCH_LAUNCH(1) break;
+        case 2: WFD_BATCH_LAUNCH(2) break;
+        case 4: WFD_BATCH_LAUNCH(4) break;
+        default: WFD_BATCH_LAUNCH(8) break;
+      }
+#undef WFD_BATCH_LAUNCH
+    }
+  }
+  WFD_LAUNCH_CHECK("batch_step");
   return WFD_OK;
 }
 
