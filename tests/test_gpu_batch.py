@@ -67,3 +67,29 @@ def test_batch_validation(pkg, gpu, cases):
         pkg.batch.Batch([s[0], packed])
     with pytest.raises(ValueError):
         pkg.batch.Batch(s, graph_steps=32)
+
+
+def test_batched_forward_adjoint_equals_single_problems(pkg, gpu, tmp_path):
+    """BatchTransmission (batched forwards with checkpoints between chunk
+    graphs, per-design reverses) equals each design's own
+    TransmissionProblem bit for bit: T and dT/drho."""
+    rng = np.random.default_rng(7)
+    cases, rhos = [], []
+    for k, (shape, side) in enumerate((((48, 48, 24), 12), ((56, 48, 24), 14),
+                                       ((48, 56, 32), 12))):
+        rho = pkg.cases.design_density(rng, (side, side), blurred=bool(k % 2))
+        cases.append(pkg.cases.bend_case(tmp_path / f"b{k}", rho, shape=shape, spacing=40e-9,
+                                         steps=STEPS, pml=6, design=(side, side),
+                                         pulse_duration=20e-15,
+                                         flux_wavelengths=[1.5e-6, 1.55e-6, 1.6e-6]))
+        rhos.append(rng.uniform(0, 1, (side, side)))
+    bt = pkg.design.BatchTransmission(cases, wavelength=1.55e-6, checkpoint_every=128)
+    T = bt.forward(rhos)
+    grads = bt.backward()
+    for case, rho, t, g in zip(cases, rhos, T, grads):
+        p = pkg.design.TransmissionProblem(case, wavelength=1.55e-6, checkpoint_every=128)
+        t1, g1 = p.value_and_grad(rho)
+        assert t == t1
+        np.testing.assert_array_equal(g.cpu().numpy(), g1.cpu().numpy())
+    with pytest.raises(ValueError):
+        pkg.design.BatchTransmission(cases, checkpoint_every=100)

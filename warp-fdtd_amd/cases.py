@@ -18,6 +18,7 @@ planes) and ``bend_case`` (200x200x80 bend with a 100x100 design region).
 
 import hashlib
 import json
+import os
 from pathlib import Path
 
 import numpy as np
@@ -229,10 +230,13 @@ def waveguide_case(path, shape=(160, 96, 80), spacing=20e-9, steps=4000, pml=12,
 
 def bend_case(path, rho, shape=(200, 200, 80), spacing=20e-9, steps=4000, pml=10,
               courant_factor=0.99, wavelength=1.55e-6, design=(100, 100),
-              wavelengths=None, pulse_duration=60e-15):
+              wavelengths=None, pulse_duration=60e-15, flux_wavelengths=None):
     """90-degree bend: a 500x220 nm guide enters along +x, leaves along +y;
     the 2x2 um design region between them holds eps = 2.085 + rho*(12.08-2.085)
-    through the 220 nm layer.  rho has shape ``design``."""
+    through the 220 nm layer.  rho has shape ``design``.  Modal DFT planes
+    mode_in / mode_out record ``wavelengths`` (default: the source's);
+    ``flux_wavelengths`` adds flux-only DFT planes flux_in / flux_out on the
+    same planes (no mode solve)."""
     nx, ny, nz = shape
     rho = np.asarray(rho, np.float64)
     if rho.shape != tuple(design):
@@ -268,4 +272,75 @@ def bend_case(path, rho, shape=(200, 200, 80), spacing=20e-9, steps=4000, pml=10
                ins, spacing, dt)
     _dft_plane(meta, payload, "mode_out", [[0, nx], [yo, yo + 1], [0, nz]], 1, wavelengths,
                outs, spacing, dt)
+    if flux_wavelengths is not None:
+        _dft_plane(meta, payload, "flux_in", [[sx + 8, sx + 9], [0, ny], [0, nz]], 0,
+                   flux_wavelengths)
+        _dft_plane(meta, payload, "flux_out", [[0, nx], [yo, yo + 1], [0, nz]], 1,
+                   flux_wavelengths)
     return write_bundle(path, meta, payload)
+
+
+def design_density(rng, shape, blurred, sigma=2.0):
+    """A seeded density in [0, 1]: uniform, or uniform blurred by a Gaussian of
+    ``sigma`` voxels (reflecting edges) and stretched back to [0, 1]."""
+    rho = rng.uniform(0, 1, shape)
+    if blurred:
+        from scipy.ndimage import gaussian_filter
+
+        rho = gaussian_filter(rho, sigma, mode="reflect")
+        lo, hi = rho.min(), rho.max()
+        rho = (rho - lo) / max(hi - lo, 1e-12)
+    return rho
+
+
+def batch_shapes(count=32, seed=0, lo=(120, 120, 64), hi=(256, 224, 96), pml=10):
+    """Seeded uneven grid shapes in [lo, hi] (per axis, multiples of 8) with
+    each design's in-plane voxel count: 100, or less where the grid leaves
+    under 20 cells between the design region and the CPML."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        shape = tuple(int(8 * rng.integers(a // 8, b // 8 + 1)) for a, b in zip(lo, hi))
+        side = int(min(100, min(shape[:2]) - 2 * pml - 40))
+        blurred = bool(rng.integers(0, 2))
+        out.append((shape, side, blurred, int(rng.integers(0, 2 ** 31))))
+    return out
+
+
+def _batch_member(args):
+    path, shape, side, blurred, seed, steps, pml, wavelength, flux_wavelengths = args
+    rho = design_density(np.random.default_rng(seed), (side, side), blurred)
+    bend_case(path, rho, shape=shape, steps=steps, pml=pml, design=(side, side),
+              wavelength=wavelength, flux_wavelengths=flux_wavelengths)
+    return str(path), rho
+
+
+def batch_cases(root, count=32, seed=0, steps=3000, pml=10, wavelength=1.55e-6,
+                flux_wavelengths=None, workers=None):
+    """The batched-design workload: ``count`` bends of seeded uneven shapes
+    (120x120x64 .. 256x224x96), densities uniform or Gaussian-blurred
+    (sigma 2 voxels), a modal objective at ``wavelength`` and 8-wavelength
+    flux DFT planes (1.50-1.60 um).  Bundles are written in parallel
+    processes and reused when present.  Returns [(path, rho)]."""
+    from concurrent.futures import ProcessPoolExecutor
+    from pathlib import Path
+
+    if flux_wavelengths is None:
+        flux_wavelengths = list(np.linspace(1.50e-6, 1.60e-6, 8))
+    root = Path(root)
+    jobs = []
+    for k, (shape, side, blurred, s) in enumerate(batch_shapes(count, seed, pml=pml)):
+        name = f"design{k:02d}_{shape[0]}x{shape[1]}x{shape[2]}_{side}_{int(blurred)}"
+        jobs.append((root / name, shape, side, blurred, s, steps, pml, wavelength,
+                     flux_wavelengths))
+    todo = [j for j in jobs if not (j[0] / "manifest.json").exists()]
+    if todo:
+        workers = workers or min(16, len(todo), os.cpu_count() or 1)
+        with ProcessPoolExecutor(max_workers=workers) as pool:
+            list(pool.map(_batch_member, todo))
+    out = []
+    for j in jobs:
+        path, shape, side, blurred, s = j[:5]
+        rho = design_density(np.random.default_rng(s), (side, side), blurred)
+        out.append((path, rho))
+    return out

@@ -20,7 +20,6 @@ import torch
 
 from ._lib import call, ptr, stream
 from .adjoint import Adjoint
-from .array import Array
 
 EPS_OXIDE = 2.085
 EPS_SI = 12.08
@@ -157,3 +156,77 @@ class Transmission(torch.autograd.Function):
 def transmission(rho, problem):
     """Differentiable T(rho) (torch.autograd)."""
     return Transmission.apply(rho, problem)
+
+
+class BatchTransmission:
+    """Forward + adjoint of many density designs of uneven grids.
+
+    The forwards of every design run together: one ``Batch`` graph per chunk
+    steps all grids, and each design's field checkpoint is saved between
+    chunks every ``checkpoint_every`` steps (a multiple of ``graph_steps``).
+    Each design's reverse then runs on its own recompute/reverse graphs.
+    Values and gradients are bitwise those of the designs' own
+    ``TransmissionProblem``s.
+    """
+
+    def __init__(self, cases, wavelength=None, checkpoint_every=192, graph_steps=64,
+                 device="cuda:0"):
+        from .batch import Batch
+
+        if checkpoint_every is None or checkpoint_every % graph_steps:
+            raise ValueError("checkpoint_every must be a multiple of graph_steps")
+        self.problems = [TransmissionProblem(c, wavelength=wavelength,
+                                             checkpoint_every=checkpoint_every, device=device,
+                                             graph_steps=graph_steps, resident=False)
+                         for c in cases]
+        self.batch = Batch([p.adjoint for p in self.problems], graph_steps=graph_steps)
+        self.checkpoint_every = checkpoint_every
+        self.graph_steps = graph_steps
+        self.cells = self.batch.cells
+        self.steps = self.batch.steps
+        self._compiled = False
+        self.timings = {}
+
+    def compile(self):
+        if not self._compiled:
+            for p in self.problems:
+                p._compile()
+            self.batch.compile()
+            self._compiled = True
+
+    def forward(self, rhos):
+        """T of every design (list of floats) for the densities ``rhos``."""
+        if len(rhos) != len(self.problems):
+            raise ValueError("one density per design")
+        self.compile()
+        t0 = time.perf_counter()
+        written = [p.map.write(r, p.adjoint) for p, r in zip(self.problems, rhos)]
+        self.batch.reset()
+        active = np.zeros(self.steps, bool)
+        for p in self.problems:
+            active |= p.adjoint._windows()
+        K, G = self.checkpoint_every, self.graph_steps
+        s = stream()
+        for c, start in enumerate(range(0, self.steps, K)):
+            for p in self.problems:
+                for dst, src in zip(p.adjoint.checkpoints[c], p.adjoint._state()):
+                    dst.tensor.copy_(src.tensor)
+            t, end = start, min(start + K, self.steps)
+            while t < end:
+                length = min(G, end - t)
+                g = (self.batch.graphs if active[t:t + length].any()
+                     else self.batch.quiet_graphs)[length]
+                call("wfd_graph_launch", g, s)
+                t += length
+        torch.cuda.synchronize(self.problems[0].adjoint.device)
+        self.timings["forward_s"] = time.perf_counter() - t0
+        for p, r in zip(self.problems, written):
+            p._forward_rho = r.clone()
+        return [p.adjoint.transmission() for p in self.problems]
+
+    def backward(self):
+        """dT/drho of every design for the densities of the last forward."""
+        t0 = time.perf_counter()
+        grads = [p.backward(p._forward_rho) for p in self.problems]
+        self.timings["backward_s"] = time.perf_counter() - t0
+        return grads
