@@ -332,6 +332,45 @@ int wfd_density_forward(const double* rho, float* inv_eps, int nx, int ny, int n
 int wfd_density_pullback(const double* rho, const double* grad_inv_eps, double* grad_rho, int sx,
                          int sy, int sz, double eps_lo, double eps_hi, void* stream);
 
+/* ------------------------------------------- Fourier-neural-operator kernels
+ * No reference counterpart (the surrogate of the density-design loop).
+ * gemm_bf16: C[b] = alpha * A[b] (m x k) * B[b] (k x n) + beta * C[b] for
+ * b < batch, element (b, i, j) of each operand at base + b*s?b + i*s?(row) +
+ * j*s?(col) (strides in elements); A and B are FP32 (a_bf16/b_bf16 = 0) or
+ * BF16 bit patterns (1), rounded to BF16 and multiplied on the matrix cores
+ * with FP32 accumulation; C is FP32.  Transposes are stride swaps. */
+int wfd_gemm_bf16(const void* a, int a_bf16, long long sab, long long sam, long long sak,
+                  const void* b, int b_bf16, long long sbb, long long sbk, long long sbn,
+                  float* c, long long scb, long long scm, long long scn, int batch, int m, int n,
+                  int k, float alpha, float beta, void* stream);
+/* y = BF16 bits of x (round to nearest even); round_bf16 keeps FP32 storage. */
+int wfd_to_bf16(const float* x, void* y, long long n, void* stream);
+int wfd_round_bf16(const float* x, float* y, long long n, void* stream);
+/* pre = y + bias[j] over rows of n (bias may be NULL, pre may be NULL);
+ * out = pre (act 0) or GELU(pre) (act 1, erf form).  act_back: dz = dout * act'(pre). */
+int wfd_bias_act(const float* y, const float* bias, float* pre, float* out, long long rows,
+                 int n, int act, void* stream);
+int wfd_act_back(const float* dout, const float* pre, float* dz, long long total, int act,
+                 void* stream);
+/* out[j] = sum_i x[i, j] (+ beta * out[j]) in a fixed order; scratch holds
+ * wfd_colsum_scratch(rows, n) floats. */
+long long wfd_colsum_scratch(long long rows, int n);
+int wfd_colsum(const float* x, long long rows, int n, float* out, float* scratch,
+               long long scratch_floats, float beta, void* stream);
+/* y[b, j] = mean_i x[b, i, j] and its transpose (dx = dy / p + beta * dx). */
+int wfd_pool_mean(const float* x, int batch, int p, int w, float* y, void* stream);
+int wfd_pool_mean_back(const float* dy, int batch, int p, int w, float* dx, float beta,
+                       void* stream);
+/* d = scale * (pred - target); partial[k] = sum of squared errors of block k
+ * (wfd_mse_partials(n) blocks). */
+long long wfd_mse_partials(long long n);
+int wfd_mse(const float* pred, const float* target, long long n, float scale, float* d,
+            float* partial, void* stream);
+/* Adam step t >= 1 on FP32 master weights (decoupled weight decay `decay`),
+ * writing the BF16 copy w16 when non-NULL. */
+int wfd_adam(float* w, const float* g, float* m, float* v, void* w16, long long n, float lr,
+             float b1, float b2, float eps, int t, float decay, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -136,6 +136,18 @@ SIGNATURES = {
     "wfd_record_mode": [P, P, P, P],
     "wfd_material_forward": [P, P, P, P, I, I, F, P],
     "wfd_material_pullback": [P, P, P, P, P, I, I, F, P],
+    "wfd_gemm_bf16": [P, I, L, L, L, P, I, L, L, L, P, L, L, L, I, I, I, I, F, F, P],
+    "wfd_to_bf16": [P, P, L, P],
+    "wfd_round_bf16": [P, P, L, P],
+    "wfd_bias_act": [P, P, P, P, L, I, I, P],
+    "wfd_act_back": [P, P, P, L, I, P],
+    "wfd_colsum_scratch": [L, I],
+    "wfd_colsum": [P, L, I, P, P, L, F, P],
+    "wfd_pool_mean": [P, I, I, I, P, P],
+    "wfd_pool_mean_back": [P, I, I, I, P, F, P],
+    "wfd_mse_partials": [L],
+    "wfd_mse": [P, P, L, F, P, P, P],
+    "wfd_adam": [P, P, P, P, P, L, F, F, F, F, I, F, P],
     "wfd_density_forward": [P, P] + [I] * 9 + [ctypes.c_double, ctypes.c_double, P],
     "wfd_density_pullback": [P, P, P, I, I, I, ctypes.c_double, ctypes.c_double, P],
 }
@@ -155,7 +167,9 @@ def lib():
             fn = getattr(handle, name)
             fn.argtypes = args
             fn.restype = {"wfd_last_error": ctypes.c_char_p,
-                          "wfd_batch_workspace_bytes": ctypes.c_int64}.get(name, ctypes.c_int)
+                          "wfd_batch_workspace_bytes": ctypes.c_int64,
+                          "wfd_colsum_scratch": ctypes.c_int64,
+                          "wfd_mse_partials": ctypes.c_int64}.get(name, ctypes.c_int)
         _lib = handle
     return _lib
 

@@ -7,7 +7,7 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 SOURCES = ["yee.hip", "detect.hip", "adjoint.hip", "driven.hip", "material.hip",
-           "stage.hip", "resident.hip", "design.hip", "runtime.hip"]
+           "stage.hip", "resident.hip", "design.hip", "fno.hip", "runtime.hip"]
 # Every float operation of the kernels rounds on its own (no FMA contraction),
 # so they reproduce the reference's operation order bit for bit; packed FP32
 # instructions are disabled.
