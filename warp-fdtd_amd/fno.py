@@ -294,9 +294,9 @@ class FNO:
     def _wgrad(self, x, rows_per, k, dy, n, name):
         """grad[name] (k, n) = sum_b x_b^T (k x rows) . dy_b (rows x n)."""
         B = self.B
-        part = self.partials[:, :k * n]
+        part = self.partials.view(-1)[:B * k * n].view(B, k * n)
         self.gemm(x, False, (rows_per * k, 1, k), dy, False, (rows_per * n, n, 1), part,
-                  (part.stride(0), n, 1), B, k, n, rows_per)
+                  (k * n, n, 1), B, k, n, rows_per)
         self.colsum(part, B, k * n, self.gview(name).view(-1), beta=0.0)
 
     def backward(self, d_t, d_g):
