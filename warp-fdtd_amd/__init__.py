@@ -7,8 +7,8 @@ libwfdtd.so (include/wfdtd.h).  Device arrays are torch tensors on ROCm.
 
 from . import (_lib, adjoint, array, autotune, batch, cases, convergence, design, detectors,
                driven_uniform, guide_observer, guide_problem, guide_validation, kernels,
-               fno, material, modes, periodic, solver, volume)
+               fno, material, modes, periodic, solver, surrogate, volume)
 
 __all__ = ["_lib", "adjoint", "array", "autotune", "batch", "cases", "convergence", "design",
            "detectors", "driven_uniform", "guide_observer", "guide_problem", "guide_validation",
-           "fno", "kernels", "material", "modes", "periodic", "solver", "volume"]
+           "fno", "kernels", "material", "modes", "periodic", "solver", "surrogate", "volume"]

@@ -241,6 +241,12 @@ class Adjoint(Solver):
                 p[m.name] = float(abs(beta) ** 2)
         return p["mode_out"] / p["mode_in"]
 
+    def transmission_spectrum(self):
+        """T per wavelength of DFT modal planes mode_in / mode_out."""
+        from .detectors import dft_transmission
+
+        return dft_transmission(self)
+
     @staticmethod
     def _overlap(m, em, hm, area):
         """coeff with sum(coeff * phasor) = conj(alpha) (adjoint.py:343-352)."""

@@ -288,3 +288,15 @@ def measurements(solver):
         powers["modal_transmission"] = powers["mode_out"] / powers["mode_in"]
         powers["modal_reflection"] = powers["mode_reflected"] / powers["mode_in"]
     return powers
+
+
+def dft_transmission(solver, inlet="mode_in", outlet="mode_out"):
+    """Modal transmission per wavelength of two DFT planes (no reference
+    counterpart): P_out / P_in with P = |alpha|^2 of DFTMonitor.power."""
+    planes = {m.name: m for m in solver.monitors if isinstance(m, DFTMonitor)}
+    if inlet not in planes or outlet not in planes:
+        raise ValueError(f"DFT planes {inlet!r} and {outlet!r} are required")
+    a, b = planes[inlet], planes[outlet]
+    if not np.allclose(a.wavelengths, b.wavelengths, rtol=1e-12, atol=0):
+        raise ValueError("the two planes record different wavelengths")
+    return b.power() / a.power()

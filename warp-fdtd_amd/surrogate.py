@@ -1,0 +1,204 @@
+"""Surrogate-assisted inverse design of the density bend (no reference counterpart).
+
+* ``surrogate_case`` writes the bend bundle the surrogate learns: modal DFT
+  planes at 8 wavelengths (1.50-1.60 um), a side x side design region.
+* ``Evaluator`` runs the solver: batched forward + adjoint labels
+  (``BatchTransmission``: T spectrum and dT/drho at the objective
+  wavelength), single forward verifications (a plain ``Solver``, resident
+  when that is faster) and single adjoint steps.
+* ``train`` fits the FNO (fno.py) to (rho -> T spectrum, dT/drho) on the
+  library's own BF16 kernels.
+* ``optimize`` runs gradient ascent on T at the objective wavelength either
+  with the adjoint at every step (``surrogate=None``) or with surrogate
+  proposals, each re-verified by a full FDTD forward and replaced by an
+  adjoint step when the surrogate's transmission error exceeds ``tolerance``.
+"""
+
+import time
+
+import numpy as np
+import torch
+
+from . import detectors
+from .cases import bend_case, design_density
+from .design import BatchTransmission, DensityMap, _design_box
+from .solver import Solver
+
+WAVELENGTHS = np.linspace(1.50e-6, 1.60e-6, 8)
+
+
+def surrogate_case(path, side=100, spacing=40e-9, steps=1500, pml=8, margin=30, nz=24,
+                   pulse_duration=30e-15):
+    """Bend with a side x side voxel design region (spacing per voxel), the
+    arms and planes ``margin`` cells outside it, modal planes at 8 wavelengths."""
+    n = side + 2 * (pml + margin)
+    return bend_case(path, np.full((side, side), 0.5), shape=(n, n, nz), spacing=spacing,
+                     steps=steps, pml=pml, design=(side, side), wavelengths=list(WAVELENGTHS),
+                     wavelength=1.55e-6, pulse_duration=pulse_duration)
+
+
+def objective_index():
+    """The sampled wavelength nearest 1.55 um (the objective's)."""
+    return int(np.argmin(np.abs(WAVELENGTHS - 1.55e-6)))
+
+
+class Evaluator:
+    """FDTD evaluations of densities on one surrogate bundle."""
+
+    def __init__(self, case, batch=32, checkpoint_every=192, device="cuda:0"):
+        self.case = case
+        self.wavelength = float(WAVELENGTHS[objective_index()])
+        self.batch = batch
+        self.labeler = BatchTransmission([case] * batch, wavelength=self.wavelength,
+                                         checkpoint_every=checkpoint_every, device=device)
+        self.labeler.compile()
+        self.single = self.labeler.problems[0]
+        self.checker = Solver(case, device=device)
+        detectors.attach(self.checker)
+        self.checker.compile()
+        self.map = DensityMap(_design_box(case))
+        self.side = self.map.plane[0]
+        self.counts = {"forward": 0, "adjoint": 0, "labels": 0}
+
+    def label(self, rhos):
+        """(T spectra (n, 8), dT/drho (n, side, side)) for the densities."""
+        rhos = list(rhos)
+        spectra, grads = [], []
+        for start in range(0, len(rhos), self.batch):
+            chunk = rhos[start:start + self.batch]
+            pad = chunk + [chunk[-1]] * (self.batch - len(chunk))
+            self.labeler.forward(pad)
+            g = self.labeler.backward()
+            for k in range(len(chunk)):
+                spectra.append(self.labeler.problems[k].adjoint.transmission_spectrum())
+                grads.append(g[k].cpu().numpy())
+        self.counts["labels"] += len(rhos)
+        return np.array(spectra), np.array(grads)
+
+    def transmission(self, rho):
+        """T spectrum by one full FDTD forward (the verification)."""
+        s = self.checker
+        self.map.write(rho, s)
+        s.reset()
+        s.run()
+        self.counts["forward"] += 1
+        return detectors.dft_transmission(s)
+
+    def adjoint(self, rho):
+        """(T spectrum, dT/drho at the objective wavelength) by forward + adjoint."""
+        T, g = self.single.value_and_grad(rho)
+        self.counts["adjoint"] += 1
+        return self.single.adjoint.transmission_spectrum(), g.cpu().numpy()
+
+
+def random_designs(count, side, seed=0):
+    """Seeded blurred-random densities (sigma 2 voxels)."""
+    rng = np.random.default_rng(seed)
+    return [design_density(rng, (side, side), blurred=True) for _ in range(count)]
+
+
+class Normalizer:
+    def __init__(self, T, G):
+        self.t_mean = T.mean(0)
+        self.t_std = T.std(0) + 1e-6
+        self.g_scale = float(np.sqrt((G ** 2).mean())) + 1e-12
+
+
+def train(net, rhos, T, G, epochs=40, lr=2e-3, grad_weight=1.0, seed=0, norm=None, log=None):
+    """Fit the FNO to (rho -> T spectrum, dT/drho) with Adam on the GPU.
+    Returns the normalizer and the per-epoch mean loss."""
+    from ._lib import call, lib, stream
+
+    rhos = np.asarray(rhos, np.float32)
+    T = np.asarray(T, np.float64)
+    G = np.asarray(G, np.float64)
+    norm = norm or Normalizer(T, G)
+    dev = net.device
+    x = torch.from_numpy(rhos).to(dev)
+    tn = torch.from_numpy(((T - norm.t_mean) / norm.t_std).astype(np.float32)).to(dev)
+    gn = torch.from_numpy((G / norm.g_scale).astype(np.float32)).to(dev)
+    B = net.B
+    d_t = torch.zeros(B, net.F, device=dev)
+    d_g = torch.zeros(B, net.S, net.S, device=dev)
+    part_t = torch.zeros(int(lib().wfd_mse_partials(B * net.F)), device=dev)
+    part_g = torch.zeros(int(lib().wfd_mse_partials(B * net.S * net.S)), device=dev)
+    rng = np.random.default_rng(seed)
+    history = []
+    n = len(rhos)
+    for epoch in range(epochs):
+        order = rng.permutation(n)
+        total, batches = 0.0, 0
+        rate = lr * 0.5 * (1 + np.cos(np.pi * epoch / max(epochs, 1)))
+        for start in range(0, n - B + 1, B):
+            idx = torch.from_numpy(order[start:start + B]).to(dev)
+            t_out, g_out = net.forward(x.index_select(0, idx).contiguous())
+            tt = tn.index_select(0, idx).contiguous()
+            gt = gn.index_select(0, idx).contiguous()
+            call("wfd_mse", t_out.data_ptr(), tt.data_ptr(), B * net.F, 2.0 / (B * net.F),
+                 d_t.data_ptr(), part_t.data_ptr(), stream())
+            call("wfd_mse", g_out.data_ptr(), gt.data_ptr(), B * net.S * net.S,
+                 2.0 * grad_weight / (B * net.S * net.S), d_g.data_ptr(), part_g.data_ptr(),
+                 stream())
+            net.backward(d_t, d_g)
+            net.adam(rate)
+            loss = (part_t.sum() / (B * net.F) + grad_weight * part_g.sum()
+                    / (B * net.S * net.S))
+            total += float(loss)
+            batches += 1
+        history.append(total / max(batches, 1))
+        if log:
+            log(f"epoch {epoch} loss {history[-1]:.5f}")
+    return norm, history
+
+
+def predict(net, norm, rho):
+    """(T spectrum (8,), dT/drho (side, side)) of one density."""
+    x = torch.as_tensor(np.asarray(rho, np.float32), device=net.device)
+    t, g = net.forward(x[None].expand(net.B, net.S, net.S).contiguous())
+    t = t[0].detach().cpu().double().numpy() * norm.t_std + norm.t_mean
+    g = g[0].detach().cpu().double().numpy() * norm.g_scale
+    return t, g
+
+
+def _ascent(rho, g, step):
+    scale = np.abs(g).max()
+    if not np.isfinite(scale) or scale == 0:
+        return rho
+    return np.clip(rho + step * g / scale, 0.0, 1.0)
+
+
+def optimize(evaluator, rho0, steps=200, step=0.05, surrogate=None, tolerance=0.01):
+    """Gradient ascent on T at the objective wavelength.  ``surrogate`` =
+    (net, norm) proposes each step from the FNO's gradient; every proposal is
+    re-verified by a full FDTD forward, and a proposal whose surrogate T is off
+    by more than ``tolerance`` (relative) is replaced by an adjoint step.
+    Returns a dict with the T trajectory (verified), counts and wall time."""
+    f = objective_index()
+    rho = np.array(rho0, np.float64)
+    start = time.perf_counter()
+    history, fallbacks = [], 0
+    if surrogate is None:
+        for _ in range(steps):
+            spec, g = evaluator.adjoint(rho)
+            history.append(float(spec[f]))
+            rho = _ascent(rho, g, step)
+        final = float(evaluator.transmission(rho)[f])
+    else:
+        net, norm = surrogate
+        current = float(evaluator.transmission(rho)[f])
+        for _ in range(steps):
+            _, g_hat = predict(net, norm, rho)
+            proposal = _ascent(rho, g_hat, step)
+            t_hat = predict(net, norm, proposal)[0][f]
+            t_true = float(evaluator.transmission(proposal)[f])
+            if abs(t_hat - t_true) > tolerance * max(abs(t_true), 1e-12):
+                fallbacks += 1
+                spec, g = evaluator.adjoint(rho)
+                proposal = _ascent(rho, g, step)
+                t_true = float(evaluator.transmission(proposal)[f])
+            rho, current = proposal, t_true
+            history.append(current)
+        final = current
+    torch.cuda.synchronize()
+    return {"final": final, "history": history, "rho": rho, "fallbacks": fallbacks,
+            "seconds": time.perf_counter() - start, "counts": dict(evaluator.counts)}
