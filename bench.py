@@ -136,6 +136,144 @@ def copy_probe(mib):
     print(json.dumps({"copy_mib": mib, "copies": 8}), flush=True)
 
 
+def _line(metric, value, unit, steps, warmup, ms, config, dtype="f32", **extra):
+    line = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+def bench_adjoint(args):
+    """Adjoint gradients/s on the 200x200x80 bend (100x100 voxels, 4000 steps,
+    checkpoints every 64), with central differences (h = 1e-3) on 16 seeded
+    voxels as the accuracy check."""
+    import torch
+
+    import __graft_entry__
+
+    pkg = __graft_entry__.package()
+    rho = np.random.default_rng(0).uniform(0, 1, (100, 100))
+    path = Path(tempfile.gettempdir()) / "wfdtd_bench_bend_200x200x80"
+    if not (path / "manifest.json").exists():
+        pkg.cases.bend_case(path, rho, shape=(200, 200, 80), steps=4000, pml=10,
+                            design=(100, 100), wavelength=1.55e-6)
+    problem = pkg.design.TransmissionProblem(path, checkpoint_every=64)
+    problem.value_and_grad(rho)  # compiles the graphs
+    t0 = time.perf_counter()
+    T, grad = problem.value_and_grad(rho)
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - t0
+    grad = grad.cpu().numpy()
+    fd = None
+    if args.fd:
+        rng = np.random.default_rng(16)
+        picks = rng.choice(100 * 100, 16, replace=False)
+        rows = []
+        for p_ in picks:
+            i, j = divmod(int(p_), 100)
+            up, dn = rho.copy(), rho.copy()
+            up[i, j] += 1e-3
+            dn[i, j] -= 1e-3
+            rows.append(((problem.forward(up) - problem.forward(dn)) / 2e-3, grad[i, j]))
+        f, a = np.array(rows).T
+        fd = {"voxels": 16, "h": 1e-3, "rel_l2": float(np.linalg.norm(a - f) / np.linalg.norm(f)),
+              "max_rel": float(np.max(np.abs(a - f) / np.abs(f))), "fd": f.tolist(),
+              "adjoint": a.tolist(), "precision": "float32 forward runs on the GPU"}
+    n = problem.adjoint.n
+    _line("adjoint gradients/s (200x200x80 bend, 100x100 voxels, 4000 steps, checkpoints/64)",
+          round(1.0 / seconds, 4), "gradients/s", 4000, 1, round(seconds * 1e3 / 4000, 5),
+          {"workload": "90-degree Si bend 200x200x80 @ 20 nm, rho~U[0,1) seed 0, "
+                       "eps = 2.085 + rho*9.995 through 11 z-cells, TE0 T at 1.55 um",
+           "cells": n, "T": T, "forward_s": problem.timings["forward_s"],
+           "backward_s": problem.timings["backward_s"],
+           "forward_Mcell_updates_per_s": round(n * 4000 / problem.timings["forward_s"] / 1e6,
+                                                1)},
+          fd_check=fd)
+
+
+def bench_batch(args):
+    """32 uneven bend designs (120x120x64 .. 256x224x96, 3000 steps, 8-wavelength
+    flux DFT planes + a modal objective): batched forward + per-design adjoint."""
+    import torch
+
+    import __graft_entry__
+
+    pkg = __graft_entry__.package()
+    root = Path(tempfile.gettempdir()) / "wfdtd_bench_batch32"
+    t0 = time.perf_counter()
+    cases = pkg.cases.batch_cases(root, count=32, seed=0, steps=3000)
+    gen = time.perf_counter() - t0
+    bt = pkg.design.BatchTransmission([c for c, _ in cases], wavelength=1.55e-6,
+                                      checkpoint_every=192)
+    bt.compile()
+    rhos = [r for _, r in cases]
+    t0 = time.perf_counter()
+    T = bt.forward(rhos)
+    fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    bt.backward()
+    torch.cuda.synchronize()
+    bwd = time.perf_counter() - t0
+    updates = bt.cells * bt.steps
+    _line("aggregate Mcell-updates/s (32 uneven bend designs, batched forward)",
+          round(updates / fwd / 1e6, 1), "Mcell-updates/s", bt.steps, 0,
+          round(fwd * 1e3 / bt.steps, 5),
+          {"workload": "32 seeded bends 120x120x64..256x224x96 @ 20 nm, uniform or blurred "
+                       "densities, 8-wavelength flux DFT + modal planes, 3000 steps",
+           "designs": 32, "cells": bt.cells, "forward_s": round(fwd, 3),
+           "adjoint_s": round(bwd, 3), "designs_per_s_forward_adjoint": round(32 / (fwd + bwd), 4),
+           "generation_s": round(gen, 1), "T_mean": float(np.mean(T)),
+           "x_chunks": sorted({s.x_chunk for s in bt.batch.solvers})})
+
+
+def bench_surrogate(args):
+    """Surrogate-assisted inverse design on the 100x100-voxel bend at 40 nm."""
+    import torch
+
+    import __graft_entry__
+
+    pkg = __graft_entry__.package()
+    sg = pkg.surrogate
+    path = Path(tempfile.gettempdir()) / "wfdtd_bench_surrogate100"
+    if not (path / "manifest.json").exists():
+        sg.surrogate_case(path, side=100)
+    ev = sg.Evaluator(path, batch=32, checkpoint_every=192)
+    t0 = time.perf_counter()
+    rhos = sg.random_designs(args.designs, 100, seed=0)
+    T, G = ev.label(rhos)
+    label_s = time.perf_counter() - t0
+    held = sg.random_designs(64, 100, seed=1)
+    Th, Gh = ev.label(held)
+    net = pkg.fno.FNO(100, batch=32, seed=0)
+    t0 = time.perf_counter()
+    norm, hist = sg.train(net, rhos, T, G, epochs=args.epochs, lr=2e-3)
+    train_s = time.perf_counter() - t0
+    f = sg.objective_index()
+    pred = np.array([sg.predict(net, norm, r)[0] for r in held])
+    rel = np.abs(pred[:, f] - Th[:, f]) / np.abs(Th[:, f])
+    rho0 = sg.random_designs(1, 100, seed=7)[0]
+    plain = sg.optimize(ev, rho0, steps=args.opt_steps, step=0.05)
+    sur = sg.optimize(ev, rho0, steps=args.opt_steps, step=0.05, surrogate=(net, norm))
+    torch.cuda.synchronize()
+    within = float(np.mean(rel <= 0.005))
+    per_hour = args.opt_steps / sur["seconds"] * 3600
+    _line("designs/hr at <=0.5% T error (surrogate-assisted loop, FDTD-verified)",
+          round(per_hour, 1), "designs/hr", args.opt_steps, 0,
+          round(sur["seconds"] * 1e3 / args.opt_steps, 3),
+          {"workload": "bend with 100x100-voxel density design at 40 nm (176x176x24 cells, "
+                       "1500 steps), 4-layer FNO width 32, 12 modes, BF16 MFMA",
+           "training_designs": args.designs, "epochs": args.epochs, "label_s": round(label_s, 1),
+           "train_s": round(train_s, 1), "final_loss": hist[-1],
+           "heldout_T_rel_err_median": float(np.median(rel)),
+           "heldout_within_0.5pct": within,
+           "plain_final_T": plain["final"], "surrogate_final_T": sur["final"],
+           "final_T_rel_diff": abs(sur["final"] - plain["final"]) / abs(plain["final"]),
+           "plain_s": round(plain["seconds"], 2), "surrogate_s": round(sur["seconds"], 2),
+           "speedup": round(plain["seconds"] / sur["seconds"], 3),
+           "fallbacks": sur["fallbacks"]}, dtype="bf16")
+
+
 def main():
     p = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     p.add_argument("--gpus", type=int, default=1)
@@ -150,7 +288,19 @@ def main():
     p.add_argument("--copy-mib", type=int, default=0,
                    help="only run 8 device-to-device copies of this many MiB (the reference "
                         "traffic for calibrating HBM counters) and exit")
+    p.add_argument("--workload", choices=("waveguide", "bend-adjoint", "batch", "surrogate"),
+                   default="waveguide",
+                   help="waveguide: the headline Yee-step metric; the others measure the "
+                        "adjoint, batched-design and surrogate workloads")
+    p.add_argument("--fd", action="store_true", help="bend-adjoint: central-difference check")
+    p.add_argument("--designs", type=int, default=1024, help="surrogate: training designs")
+    p.add_argument("--epochs", type=int, default=40, help="surrogate: training epochs")
+    p.add_argument("--opt-steps", type=int, default=200, help="surrogate: optimisation steps")
     args = p.parse_args()
+    if args.workload != "waveguide" and args.gpus == 1:
+        {"bend-adjoint": bench_adjoint, "batch": bench_batch,
+         "surrogate": bench_surrogate}[args.workload](args)
+        return
     if args.copy_mib:
         copy_probe(args.copy_mib)
         return
