@@ -81,7 +81,8 @@ def test_elementwise_and_reductions(pkg, gpu):
     assert (out.cpu().double() - gelu).abs().max() < 1e-5
     dout = torch.randn(rows, n, generator=g)
     dz = torch.empty(rows, n).cuda()
-    lib.call("wfd_act_back", P(dout.cuda().data_ptr()), P(pre.data_ptr()), P(dz.data_ptr()),
+    dout_d = dout.cuda()
+    lib.call("wfd_act_back", P(dout_d.data_ptr()), P(pre.data_ptr()), P(dz.data_ptr()),
              rows * n, 1, s)
     zz = z.clone().requires_grad_()
     (0.5 * zz * (1 + torch.erf(zz / np.sqrt(2)))).backward(dout.double())
@@ -95,13 +96,15 @@ def test_elementwise_and_reductions(pkg, gpu):
     assert (cs.cpu().double() - (y.double().sum(0) + 1)).abs().max() < 1e-3
     x = torch.randn(4, 250, 9, generator=g)
     pooled = torch.empty(4, 9).cuda()
-    lib.call("wfd_pool_mean", P(x.cuda().data_ptr()), 4, 250, 9, P(pooled.data_ptr()), s)
+    x_d = x.cuda()
+    lib.call("wfd_pool_mean", P(x_d.data_ptr()), 4, 250, 9, P(pooled.data_ptr()), s)
     torch.cuda.synchronize()
     assert (pooled.cpu().double() - x.double().mean(1)).abs().max() < 1e-6
     pred, target = torch.randn(1000, generator=g), torch.randn(1000, generator=g)
     d = torch.empty(1000).cuda()
     parts = torch.empty(lib.lib().wfd_mse_partials(1000)).cuda()
-    lib.call("wfd_mse", P(pred.cuda().data_ptr()), P(target.cuda().data_ptr()), 1000, 0.002,
+    pred_d, target_d = pred.cuda(), target.cuda()
+    lib.call("wfd_mse", P(pred_d.data_ptr()), P(target_d.data_ptr()), 1000, 0.002,
              P(d.data_ptr()), P(parts.data_ptr()), s)
     torch.cuda.synchronize()
     assert abs(parts.cpu().double().sum() - ((pred - target).double() ** 2).sum()) < 1e-3
@@ -110,7 +113,8 @@ def test_elementwise_and_reductions(pkg, gpu):
     gr = torch.randn(500, generator=g)
     wd, m, v = w.cuda(), torch.zeros(500).cuda(), torch.zeros(500).cuda()
     w16 = torch.empty(500, dtype=torch.bfloat16).cuda()
-    lib.call("wfd_adam", P(wd.data_ptr()), P(gr.cuda().data_ptr()), P(m.data_ptr()),
+    gr_d = gr.cuda()
+    lib.call("wfd_adam", P(wd.data_ptr()), P(gr_d.data_ptr()), P(m.data_ptr()),
              P(v.data_ptr()), P(w16.data_ptr()), 500, 1e-2, 0.9, 0.999, 1e-8, 1, 0.0, s)
     ref = w.clone().requires_grad_()
     opt = torch.optim.Adam([ref], lr=1e-2)

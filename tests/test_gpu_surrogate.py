@@ -20,12 +20,12 @@ def setup(tmp_path_factory, pkg):
                              spacing=40e-9, steps=700, pml=6, margin=12, nz=24,
                              pulse_duration=20e-15)
     ev = sg.Evaluator(case, batch=16, checkpoint_every=128)
-    train_rho = sg.random_designs(192, SIDE, seed=1)
+    train_rho = sg.random_designs(512, SIDE, seed=1)
     test_rho = sg.random_designs(32, SIDE, seed=2)
     T, G = ev.label(train_rho)
     Tt, Gt = ev.label(test_rho)
     net = pkg.fno.FNO(SIDE, batch=16, seed=0)
-    norm, history = sg.train(net, train_rho, T, G, epochs=60, lr=3e-3)
+    norm, history = sg.train(net, train_rho, T, G, epochs=80, lr=3e-3)
     return sg, ev, net, norm, history, test_rho, Tt, Gt
 
 

@@ -323,6 +323,9 @@ class Adjoint(Solver):
     def compile_reverse(self):
         if self.cpml_storage != "full":
             raise ValueError("The adjoint needs the full CPML storage")
+        if not self.adjoint_sources:
+            raise ValueError("prepare_sources() must run before compile_reverse(): the "
+                             "reverse graphs capture the cotangent buffers")
         self.reset_reverse()
         self.reverse_one()
         torch.cuda.synchronize(self.device)

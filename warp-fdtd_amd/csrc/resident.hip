@@ -23,6 +23,7 @@ namespace {
 constexpr int kThreads = 1024;
 constexpr int kLdsFloats = 38912;           // 152 KB of state per workgroup
 constexpr int kResidentCells = kLdsFloats / 7;  // E (3) + H (3) + 1/eps (1) per cell
+constexpr int kCoeffRows = 336;  // sx + sy + sz of a sub-box (coefficient rows in LDS)
 
 struct ResArgs {
   float* e;
@@ -48,16 +49,18 @@ struct ResArgs {
   int spin_limit;
 };
 
-// Make this workgroup's global stores visible at agent scope, then raise its
-// flag to v.
+// Raise this workgroup's flag to v once its face stores are visible at agent
+// scope: every thread waits for its own stores, the barrier orders them
+// before the release store of one thread (one L2 write-back per workgroup).
 __device__ __forceinline__ void publish(int* flag, int v) {
-  __threadfence();
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) __hip_atomic_store(flag, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Wait until the flags of up to three neighbours reach v (bounded).  Returns
-// false, uniformly across the workgroup, once the error word is raised.
+// Wait until the flags of up to three neighbours reach v (bounded polls with
+// relaxed loads, then one acquire fence).  Returns false, uniformly across the
+// workgroup, once the error word is raised.
 __device__ __forceinline__ bool wait_for(const ResArgs& a, const int* flags, int n0, int n1,
                                          int n2, int v, int* bail) {
   if (threadIdx.x == 0) {
@@ -66,16 +69,17 @@ __device__ __forceinline__ bool wait_for(const ResArgs& a, const int* flags, int
     for (int k = 0; k < 3 && !failed; ++k) {
       if (nb[k] < 0) continue;
       int spins = 0;
-      while (__hip_atomic_load(flags + nb[k], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < v) {
+      while (__hip_atomic_load(flags + nb[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < v) {
         if (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
             ++spins > a.spin_limit) {
           __hip_atomic_store(a.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           failed = 1;
           break;
         }
-        __builtin_amdgcn_s_sleep(2);
+        __builtin_amdgcn_s_sleep(1);
       }
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     *bail = failed;
   }
   __syncthreads();
@@ -94,6 +98,7 @@ __device__ __forceinline__ float cpml_step(float d, float a, float b, float k,
 
 __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
   __shared__ float st[kLdsFloats];
+  __shared__ float co[6 * kCoeffRows];  // (a, b, 1/kappa) of E then H, rows x | y | z
   __shared__ int bail;
   const int tid = threadIdx.x;
   const int wg = blockIdx.x;
@@ -101,19 +106,19 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
   const int bzi = wg % a.gz, byi = (wg / a.gz) % a.gy, bxi = wg / (a.gz * a.gy);
   const int X0 = bxi * a.sx, Y0 = byi * a.sy, Z0 = bzi * a.sz;
   const int vx = min(a.sx, a.nx - X0), vy = min(a.sy, a.ny - Y0), vz = min(a.sz, a.nz - Z0);
-  const int sy = a.sy, sz = a.sz;
-  const int cells = a.sx * sy * sz;  // LDS slots (the box may be partly outside)
-  float* E = st;                     // [3][cells]
-  float* H = st + 3 * cells;         // [3][cells]
-  float* IE = st + 6 * cells;        // [cells]
-  const int nx = a.nx, ny = a.ny, nz = a.nz, P = ny * nz, n = a.n;
+  const int sx = a.sx, sy = a.sy, sz = a.sz;
+  const int cells = sx * sy * sz;  // LDS slots (the box may be partly outside)
+  float* E = st;                   // [3][cells]
+  float* H = st + 3 * cells;       // [3][cells]
+  float* IE = st + 6 * cells;      // [cells]
+  const int ny = a.ny, nz = a.nz, P = ny * nz, n = a.n;
   const int sxy = a.gz * a.gy;
   const int wxl = bxi > 0 ? wg - sxy : -1, wxh = bxi + 1 < a.gx ? wg + sxy : -1;
   const int wyl = byi > 0 ? wg - a.gz : -1, wyh = byi + 1 < a.gy ? wg + a.gz : -1;
   const int wzl = bzi > 0 ? wg - 1 : -1, wzh = bzi + 1 < a.gz ? wg + 1 : -1;
   // face layout per workgroup: H high faces (x: 2*sy*sz, y: 2*sx*sz, z: 2*sx*sy),
   // then E low faces of the same sizes.
-  const int fyz = sy * sz, fxz = a.sx * sz, fxy = a.sx * sy;
+  const int fyz = sy * sz, fxz = sx * sz, fxy = sx * sy;
   const size_t half = 2 * (size_t)(fyz + fxz + fxy);
   const size_t per = 2 * half;
   int* flagE = a.flags;
@@ -122,16 +127,45 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
   const int t0 = a.counters[0];
   const float S = a.courant;
   const int rows = a.rows;
-  const float* __restrict__ ce = a.coeff_e;
-  const float* __restrict__ ch = a.coeff_h;
+  // coefficient rows of this sub-box: E at co[0..3R), H at co[3R..6R); within a
+  // field x rows at [0, sx), y at [sx, sx+sy), z at [sx+sy, sx+sy+sz), 3 floats each
+  const int R = sx + sy + sz;
+  for (int r = tid; r < 2 * R; r += kThreads) {
+    const int f = r / R, q = r % R;
+    const float* src = f ? a.coeff_h : a.coeff_e;
+    int axis, pos;
+    if (q < sx) {
+      axis = 0;
+      pos = min(X0 + q, a.nx - 1);
+    } else if (q < sx + sy) {
+      axis = 1;
+      pos = min(Y0 + q - sx, ny - 1);
+    } else {
+      axis = 2;
+      pos = min(Z0 + q - sx - sy, nz - 1);
+    }
+    for (int k = 0; k < 3; ++k) co[(f * R + q) * 3 + k] = src[(axis * rows + pos) * 3 + k];
+  }
+  const float* cE = co;
+  const float* cH = co + 3 * R;
+
+  // this thread's cells: slot q holds cell c = tid + q*kThreads, c = (lx*sy + ly)*sz + lz
+  const float inv_sz = 1.0f / (float)sz, inv_sy = 1.0f / (float)sy;
+  auto coords = [&](int c, int& lx, int& ly, int& lz) {
+    const int r = div_small(c, sz, inv_sz);
+    lz = c - r * sz;
+    lx = div_small(r, sy, inv_sy);
+    ly = r - lx * sy;
+  };
 
   // load the sub-box (cells outside the grid hold zeros)
+#pragma unroll 1
   for (int c = tid; c < cells; c += kThreads) {
-    const int lz = c % sz, ly = (c / sz) % sy, lx = c / (sz * sy);
-    const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
+    int lx, ly, lz;
+    coords(c, lx, ly, lz);
     float v[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (lx < vx && ly < vy && lz < vz) {
-      const int i = (x * ny + y) * nz + z;
+      const int i = ((X0 + lx) * ny + Y0 + ly) * nz + Z0 + lz;
       v[0] = a.e[i];
       v[1] = a.e[n + i];
       v[2] = a.e[2 * n + i];
@@ -148,26 +182,33 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
   }
   __syncthreads();
 
-  auto publish_h = [&](int v) {
-    float* f = a.face + (size_t)wg * per;
-    for (int c = tid; c < cells; c += kThreads) {
-      const int lz = c % sz, ly = (c / sz) % sy, lx = c / (sz * sy);
-      if (lx == vx - 1) {
-        f[ly * sz + lz] = H[cells + c];
-        f[fyz + ly * sz + lz] = H[2 * cells + c];
-      }
-      if (ly == vy - 1) {
-        f[2 * fyz + lx * sz + lz] = H[c];
-        f[2 * fyz + fxz + lx * sz + lz] = H[2 * cells + c];
-      }
-      if (lz == vz - 1) {
-        f[2 * fyz + 2 * fxz + lx * sy + ly] = H[c];
-        f[2 * fyz + 2 * fxz + fxy + lx * sy + ly] = H[cells + c];
-      }
+  // faces: x (lx fixed) over (ly, lz), y over (lx, lz), z over (lx, ly)
+  auto write_faces = [&](float* f, const float* F, int fx_, int fy_, int fz_) {
+    const float* F0 = F;
+    const float* F1 = F + cells;
+    const float* F2 = F + 2 * cells;
+    for (int t = tid; t < vy * vz; t += kThreads) {
+      const int ly = t / vz, lz = t - ly * vz;
+      const int c = (fx_ * sy + ly) * sz + lz;
+      f[ly * sz + lz] = F1[c];
+      f[fyz + ly * sz + lz] = F2[c];
     }
-    publish(flagH + wg, v);
+    for (int t = tid; t < vx * vz; t += kThreads) {
+      const int lx = t / vz, lz = t - lx * vz;
+      const int c = (lx * sy + fy_) * sz + lz;
+      f[2 * fyz + lx * sz + lz] = F0[c];
+      f[2 * fyz + fxz + lx * sz + lz] = F2[c];
+    }
+    for (int t = tid; t < vx * vy; t += kThreads) {
+      const int lx = t / vy, ly = t - lx * vy;
+      const int c = (lx * sy + ly) * sz + fz_;
+      f[2 * fyz + 2 * fxz + lx * sy + ly] = F0[c];
+      f[2 * fyz + 2 * fxz + fxy + lx * sy + ly] = F1[c];
+    }
   };
-  publish_h(1);
+  float* myface = a.face + (size_t)wg * per;
+  write_faces(myface, H, vx - 1, vy - 1, vz - 1);
+  publish(flagH + wg, 1);
 
   for (int k = 0; k < nsteps; ++k) {
     const int t = t0 + k;
@@ -180,8 +221,10 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
       const bool src_here = a.source_x >= X0 && a.source_x < X0 + vx && t < a.steps;
       const float wave_y = src_here ? a.wave_h[2 * a.steps + t] : 0.f;
       const float wave_z = src_here ? a.wave_h[a.steps + t] : 0.f;
+#pragma unroll 1
       for (int c = tid; c < cells; c += kThreads) {
-        const int lz = c % sz, ly = (c / sz) % sy, lx = c / (sz * sy);
+        int lx, ly, lz;
+        coords(c, lx, ly, lz);
         if (lx >= vx || ly >= vy || lz >= vz) continue;
         const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
         const int i = (x * ny + y) * nz + z;
@@ -217,9 +260,9 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
         const float dx1 = hy - my, dx2 = hz - mz;
         const float dy0 = hx - yx, dy2 = hz - yz;
         const float dz0 = hx - zx, dz1 = hy - zy;
-        const float* cx = ce + x * 3;
-        const float* cy = ce + (rows + y) * 3;
-        const float* cz = ce + (2 * rows + z) * 3;
+        const float* cx = cE + lx * 3;
+        const float* cy = cE + (sx + ly) * 3;
+        const float* cz = cE + (sx + sy + lz) * 3;
         float* psi = a.psi_e + i;
         const float dxz = cpml_step(dx2, cx[0], cx[1], cx[2], psi);
         const float dxy = cpml_step(dx1, cx[0], cx[1], cx[2], psi + n);
@@ -242,25 +285,8 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
       }
     }
     __syncthreads();
-    {  // low E faces for the low-side neighbours' H half
-      float* f = a.face + (size_t)wg * per + half;
-      for (int c = tid; c < cells; c += kThreads) {
-        const int lz = c % sz, ly = (c / sz) % sy, lx = c / (sz * sy);
-        if (lx == 0) {
-          f[ly * sz + lz] = E[cells + c];
-          f[fyz + ly * sz + lz] = E[2 * cells + c];
-        }
-        if (ly == 0) {
-          f[2 * fyz + lx * sz + lz] = E[c];
-          f[2 * fyz + fxz + lx * sz + lz] = E[2 * cells + c];
-        }
-        if (lz == 0) {
-          f[2 * fyz + 2 * fxz + lx * sy + ly] = E[c];
-          f[2 * fyz + 2 * fxz + fxy + lx * sy + ly] = E[cells + c];
-        }
-      }
-      publish(flagE + wg, k + 1);
-    }
+    write_faces(myface + half, E, 0, 0, 0);  // low E faces for the low neighbours' H half
+    publish(flagE + wg, k + 1);
     // ------------------------------------------------------------ H half
     if (!wait_for(a, flagE, wxh, wyh, wzh, k + 1, &bail)) return;
     {
@@ -271,8 +297,10 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
       const float wave_y = src_here ? a.wave_e[2 * a.steps + t] : 0.f;
       const float wave_z = src_here ? a.wave_e[a.steps + t] : 0.f;
       const int slot = a.nstage ? t % a.slots : 0;
+#pragma unroll 1
       for (int c = tid; c < cells; c += kThreads) {
-        const int lz = c % sz, ly = (c / sz) % sy, lx = c / (sz * sy);
+        int lx, ly, lz;
+        coords(c, lx, ly, lz);
         if (lx >= vx || ly >= vy || lz >= vz) continue;
         const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
         const int i = (x * ny + y) * nz + z;
@@ -308,9 +336,9 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
         const float dx1 = py - ey, dx2 = pz - ez;
         const float dy0 = qx - ex, dy2 = qz - ez;
         const float dz0 = rx - ex, dz1 = ry - ey;
-        const float* cx = ch + x * 3;
-        const float* cy = ch + (rows + y) * 3;
-        const float* cz = ch + (2 * rows + z) * 3;
+        const float* cx = cH + lx * 3;
+        const float* cy = cH + (sx + ly) * 3;
+        const float* cz = cH + (sx + sy + lz) * 3;
         float* psi = a.psi_h + i;
         const float dxz = cpml_step(dx2, cx[0], cx[1], cx[2], psi);
         const float dxy = cpml_step(dx1, cx[0], cx[1], cx[2], psi + n);
@@ -350,10 +378,13 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
       }
     }
     __syncthreads();
-    publish_h(k + 2);
+    write_faces(myface, H, vx - 1, vy - 1, vz - 1);
+    publish(flagH + wg, k + 2);
   }
+#pragma unroll 1
   for (int c = tid; c < cells; c += kThreads) {
-    const int lz = c % sz, ly = (c / sz) % sy, lx = c / (sz * sy);
+    int lx, ly, lz;
+    coords(c, lx, ly, lz);
     if (lx < vx && ly < vy && lz < vz) {
       const int i = ((X0 + lx) * ny + Y0 + ly) * nz + Z0 + lz;
       for (int k = 0; k < 3; ++k) {
@@ -400,7 +431,7 @@ bool choose(int nx, int ny, int nz, int capacity, int want_sx, int want_sy, int 
         gy = (ny + ey - 1) / ey;
         gz = (nz + ez - 1) / ez;
         const long blocks = (long)gx * gy * gz;
-        if (blocks > capacity) continue;
+        if (blocks > capacity || ex + ey + ez > kCoeffRows) continue;
         const long key[3] = {((long)ex * ey * ez + kThreads - 1) / kThreads,
                              (long)ex * ey + (long)ex * ez + (long)ey * ez, blocks};
         const bool better = !found || key[0] < best_key[0] ||
@@ -483,6 +514,7 @@ int wfd_resident_run(const wfd_yee_plan* p, const wfd_resident_layout* L, void* 
               WFD_MAX_STAGES);
   WFD_REQUIRE(p->nstage == 0 || p->slots > 0, "resident_run: stage slots must be positive");
   WFD_REQUIRE(L->sx > 0 && L->sy > 0 && L->sz > 0 && L->sx * L->sy * L->sz <= kResidentCells &&
+                  L->sx + L->sy + L->sz <= kCoeffRows &&
                   L->gx == (p->nx + L->sx - 1) / L->sx &&
                   L->gy == (p->ny + L->sy - 1) / L->sy &&
                   L->gz == (p->nz + L->sz - 1) / L->sz && L->blocks == L->gx * L->gy * L->gz &&

@@ -58,8 +58,9 @@ class DensityMap:
         if g.dtype != torch.float64 or g.numel() != int(np.prod(self.box[3:])):
             raise ValueError("grad_inv_eps must be float64 over the design box")
         r = self._rho(rho, g.device)
+        g = g.contiguous()
         out = torch.empty(self.plane, dtype=torch.float64, device=g.device)
-        call("wfd_density_pullback", ptr(r), ptr(g.contiguous()), ptr(out), *self.box[3:],
+        call("wfd_density_pullback", ptr(r), ptr(g), ptr(out), *self.box[3:],
              self.eps_lo, self.eps_hi, stream())
         return out
 
@@ -89,7 +90,6 @@ class TransmissionProblem:
     def _compile(self):
         if not self._compiled:
             self.adjoint.compile()
-            self.adjoint.compile_reverse()
             self._compiled = True
 
     def forward(self, rho):
@@ -113,6 +113,10 @@ class TransmissionProblem:
         a = self.adjoint
         t0 = time.perf_counter()
         a.prepare_sources()
+        if not a.reverse_graphs:
+            # the reverse graphs capture the cotangent buffers, which exist
+            # once prepare_sources has run; later calls refresh them in place
+            a.compile_reverse()
         a.reset_reverse()
         a.backward()
         # the adjoint accumulates d(-T)/d(inv_eps)
