@@ -144,15 +144,16 @@ int wfd_batch_step(const wfd_yee_plan* plans, int count, const void* workspace, 
 /* ------------------------------------------------- on-chip-resident stepping
  * The same step as wfd_yee_step (bitwise), for grids whose E, H and 1/eps fit
  * in the LDS of co-resident workgroups: sub-boxes of sx x sy x sz cells (one
- * 1024-thread workgroup each, 152 KB of LDS) stay on chip for nsteps steps;
- * faces move between workgroups through the workspace under per-workgroup
- * step flags.  layout_for picks the sub-box when sx/sy/sz are 0.
- * One cooperative launch per call (replaces nsteps pairs of half-step
- * launches of solver.py:203-234); counters advance by nsteps afterwards.
+ * 1024-thread workgroup each, up to 160 KB of LDS) stay on chip for nsteps
+ * steps; faces move between workgroups through the workspace as 64-bit words
+ * (value + step tag) that readers poll.  layout_for picks the sub-box when
+ * sx/sy/sz are 0.  One cooperative launch per call (replaces nsteps pairs of
+ * half-step launches of solver.py:203-234); counters advance by nsteps.
  * layout_for validates the decomposition (WFD_ERR_UNSUPPORTED when the
- * workgroups cannot all be co-resident) and sizes the workspace, which the
- * caller allocates zeroed; its last ints hold an error word (nonzero after a
- * bounded wait timed out) that the caller reads after the run. */
+ * workgroups cannot all be co-resident) and sizes the workspace (8-byte
+ * aligned, allocated by the caller); the int after the face words is an
+ * error word, nonzero after a bounded poll gave up (the fields of that run
+ * are then invalid), which the caller reads and clears. */
 typedef struct wfd_resident_layout {
   int sx, sy, sz;        /* sub-box cells along x, y, z */
   int gx, gy, gz;        /* workgroups along each axis */

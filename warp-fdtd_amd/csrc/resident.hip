@@ -1,19 +1,32 @@
 // On-chip-resident Yee stepping for gfx950 (grids whose E, H and 1/eps fit
-// in the LDS of the co-resident workgroups: 152 KB x one workgroup per CU).
+// in the LDS of the co-resident workgroups: 160 KB x one workgroup per CU).
 //
-// The grid is cut into sub-boxes of sx x sy x sz cells (at most kResidentCells),
-// one 1024-thread workgroup each, which holds the sub-box's E, H and 1/eps in
-// LDS for a whole chunk of steps.  A half-step reads the stencil neighbours
-// from LDS and the sub-box faces of neighbouring workgroups from global face
-// buffers, guarded by per-workgroup step flags (agent-scope release/acquire).
+// The grid is cut into sub-boxes of sx x sy x sz cells, one 1024-thread
+// workgroup each, which holds the sub-box's E, H, 1/eps and CPML coefficient
+// rows in LDS for a whole chunk of steps.  A half-step reads the stencil
+// neighbours from LDS and the sub-box faces of the neighbouring workgroups
+// from global face buffers.
+//
+// Face exchange: every face value travels as one 64-bit word, the float's
+// bits in the low half and a step tag in the high half, written and read
+// with relaxed agent-scope atomics.  A reader polls each word until its tag
+// is the one it needs, so a face value and the proof that it is current
+// arrive together: no flags, no release/acquire fences and no cache
+// write-back or invalidation (those cost more than the arithmetic here).
+// Tags within a launch: H faces of the state after k steps = 2k + 1, E faces
+// after the E half of step k = 2k + 2; the words are zeroed before each
+// launch, so a tag never matches a previous launch's word.  A writer can
+// only overwrite a word its reader has consumed: overwriting needs the
+// reader's own next faces, which the reader writes after consuming.
+//
 // Inside the chunk only CPML psi (slab cells), the faces and detector staging
 // touch memory; state is loaded at the start and written back at the end.
 // The arithmetic is the fused step's (yee.hip) operation for operation, so
 // results are bitwise those of the streaming kernels.
 //
-// Termination: every wait is bounded (spin_limit polls); a workgroup that
-// times out raises the error word and every waiting workgroup checks it, so
-// all waves exit.  The launch is cooperative, which guarantees co-residency
+// Termination: every poll loop is bounded (spin_limit polls); a thread that
+// times out raises the error word, which every poller checks, and its
+// workgroup leaves; the launch is cooperative, which guarantees co-residency
 // of the whole grid or fails.
 #include "common.hpp"
 
@@ -21,9 +34,9 @@ namespace wfd {
 namespace {
 
 constexpr int kThreads = 1024;
-constexpr int kLdsFloats = 38912;           // 152 KB of state per workgroup
-constexpr int kResidentCells = kLdsFloats / 7;  // E (3) + H (3) + 1/eps (1) per cell
-constexpr int kCoeffRows = 336;  // sx + sy + sz of a sub-box (coefficient rows in LDS)
+constexpr int kLds = 40896;  // floats of LDS per workgroup (163,584 bytes)
+
+typedef unsigned long long Word;
 
 struct ResArgs {
   float* e;
@@ -43,47 +56,51 @@ struct ResArgs {
   int rows, steps, nx, ny, nz, n, source_x;
   float courant;
   int sx, sy, sz, gx, gy, gz;
-  float* face;
-  int* flags;  // [0, blocks): E flags, [blocks, 2 blocks): H flags
+  Word* face;
   int* error;
   int spin_limit;
 };
 
-// Raise this workgroup's flag to v once its face stores are visible at agent
-// scope: every thread waits for its own stores, the barrier orders them
-// before the release store of one thread (one L2 write-back per workgroup).
-__device__ __forceinline__ void publish(int* flag, int v) {
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_store(flag, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+// LDS floats a sub-box needs: E, H, 1/eps per cell, (a, b, 1/kappa) rows of E
+// and H along each axis, and two components of the three incoming faces.
+__host__ __device__ inline long lds_floats(int sx, int sy, int sz) {
+  return 7L * sx * sy * sz + 6L * (sx + sy + sz) +
+         2L * ((long)sy * sz + (long)sx * sz + (long)sx * sy);
 }
 
-// Wait until the flags of up to three neighbours reach v (bounded polls with
-// relaxed loads, then one acquire fence).  Returns false, uniformly across the
-// workgroup, once the error word is raised.
-__device__ __forceinline__ bool wait_for(const ResArgs& a, const int* flags, int n0, int n1,
-                                         int n2, int v, int* bail) {
-  if (threadIdx.x == 0) {
-    int failed = 0;
-    const int nb[3] = {n0, n1, n2};
-    for (int k = 0; k < 3 && !failed; ++k) {
-      if (nb[k] < 0) continue;
-      int spins = 0;
-      while (__hip_atomic_load(flags + nb[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < v) {
-        if (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
-            ++spins > a.spin_limit) {
-          __hip_atomic_store(a.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          failed = 1;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
+// Words of one workgroup's face buffer: H high faces then E low faces, two
+// components each, laid out x (sy*sz), y (sx*sz), z (sx*sy).
+__host__ __device__ inline size_t face_words(int sx, int sy, int sz) {
+  return 4 * ((size_t)sy * sz + (size_t)sx * sz + (size_t)sx * sy);
+}
+
+__device__ __forceinline__ Word pack(float v, unsigned tag) {
+  return ((Word)tag << 32) | (Word)__float_as_uint(v);
+}
+
+__device__ __forceinline__ void put(Word* p, float v, unsigned tag) {
+  __hip_atomic_store(p, pack(v, tag), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Poll a word until it carries `tag`; false once the error word is raised
+// or the poll limit is reached (which raises it).
+__device__ __forceinline__ bool take(const ResArgs& a, const Word* p, unsigned tag, float& v) {
+  int spins = 0;
+  for (;;) {
+    const Word w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((unsigned)(w >> 32) == tag) {
+      v = __uint_as_float((unsigned)w);
+      return true;
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    *bail = failed;
+    if (++spins > a.spin_limit) {
+      __hip_atomic_store(a.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    if ((spins & 63) == 0 &&
+        __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+      return false;
+    __builtin_amdgcn_s_sleep(1);
   }
-  __syncthreads();
-  return *bail == 0;
 }
 
 __device__ __forceinline__ float cpml_step(float d, float a, float b, float k,
@@ -97,39 +114,39 @@ __device__ __forceinline__ float cpml_step(float d, float a, float b, float k,
 }
 
 __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
-  __shared__ float st[kLdsFloats];
-  __shared__ float co[6 * kCoeffRows];  // (a, b, 1/kappa) of E then H, rows x | y | z
+  __shared__ float lds[kLds];
   __shared__ int bail;
   const int tid = threadIdx.x;
   const int wg = blockIdx.x;
-  const int blocks = gridDim.x;
   const int bzi = wg % a.gz, byi = (wg / a.gz) % a.gy, bxi = wg / (a.gz * a.gy);
   const int X0 = bxi * a.sx, Y0 = byi * a.sy, Z0 = bzi * a.sz;
   const int vx = min(a.sx, a.nx - X0), vy = min(a.sy, a.ny - Y0), vz = min(a.sz, a.nz - Z0);
   const int sx = a.sx, sy = a.sy, sz = a.sz;
   const int cells = sx * sy * sz;  // LDS slots (the box may be partly outside)
-  float* E = st;                   // [3][cells]
-  float* H = st + 3 * cells;       // [3][cells]
-  float* IE = st + 6 * cells;      // [cells]
+  const int fyz = sy * sz, fxz = sx * sz, fxy = sx * sy;
+  const int R = sx + sy + sz;
+  float* E = lds;                 // [3][cells]
+  float* H = lds + 3 * cells;     // [3][cells]
+  float* IE = lds + 6 * cells;    // [cells]
+  float* co = lds + 7 * cells;    // [2][R][3]: E rows then H rows (x | y | z)
+  float* FX = co + 6 * R;         // incoming x face [2][sy*sz]
+  float* FY = FX + 2 * fyz;       // incoming y face [2][sx*sz]
+  float* FZ = FY + 2 * fxz;       // incoming z face [2][sx*sy]
   const int ny = a.ny, nz = a.nz, P = ny * nz, n = a.n;
   const int sxy = a.gz * a.gy;
   const int wxl = bxi > 0 ? wg - sxy : -1, wxh = bxi + 1 < a.gx ? wg + sxy : -1;
   const int wyl = byi > 0 ? wg - a.gz : -1, wyh = byi + 1 < a.gy ? wg + a.gz : -1;
   const int wzl = bzi > 0 ? wg - 1 : -1, wzh = bzi + 1 < a.gz ? wg + 1 : -1;
-  // face layout per workgroup: H high faces (x: 2*sy*sz, y: 2*sx*sz, z: 2*sx*sy),
-  // then E low faces of the same sizes.
-  const int fyz = sy * sz, fxz = sx * sz, fxy = sx * sy;
-  const size_t half = 2 * (size_t)(fyz + fxz + fxy);
-  const size_t per = 2 * half;
-  int* flagE = a.flags;
-  int* flagH = a.flags + blocks;
-  if (__hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+  const size_t per = face_words(sx, sy, sz);
+  const size_t half = per / 2;
+  Word* myface = a.face + (size_t)wg * per;
+  // a raised error word (an earlier run gave up) stops the launch, uniformly
+  if (tid == 0) bail = __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  if (bail) return;
   const int t0 = a.counters[0];
   const float S = a.courant;
   const int rows = a.rows;
-  // coefficient rows of this sub-box: E at co[0..3R), H at co[3R..6R); within a
-  // field x rows at [0, sx), y at [sx, sx+sy), z at [sx+sy, sx+sy+sz), 3 floats each
-  const int R = sx + sy + sz;
   for (int r = tid; r < 2 * R; r += kThreads) {
     const int f = r / R, q = r % R;
     const float* src = f ? a.coeff_h : a.coeff_e;
@@ -149,7 +166,7 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
   const float* cE = co;
   const float* cH = co + 3 * R;
 
-  // this thread's cells: slot q holds cell c = tid + q*kThreads, c = (lx*sy + ly)*sz + lz
+  // cell c = (lx*sy + ly)*sz + lz
   const float inv_sz = 1.0f / (float)sz, inv_sy = 1.0f / (float)sy;
   auto coords = [&](int c, int& lx, int& ly, int& lz) {
     const int r = div_small(c, sz, inv_sz);
@@ -158,7 +175,6 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
     ly = r - lx * sy;
   };
 
-  // load the sub-box (cells outside the grid hold zeros)
 #pragma unroll 1
   for (int c = tid; c < cells; c += kThreads) {
     int lx, ly, lz;
@@ -182,42 +198,70 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
   }
   __syncthreads();
 
-  // faces: x (lx fixed) over (ly, lz), y over (lx, lz), z over (lx, ly)
-  auto write_faces = [&](float* f, const float* F, int fx_, int fy_, int fz_) {
-    const float* F0 = F;
-    const float* F1 = F + cells;
-    const float* F2 = F + 2 * cells;
-    for (int t = tid; t < vy * vz; t += kThreads) {
-      const int ly = t / vz, lz = t - ly * vz;
-      const int c = (fx_ * sy + ly) * sz + lz;
-      f[ly * sz + lz] = F1[c];
-      f[fyz + ly * sz + lz] = F2[c];
+  // Outgoing faces of field F at planes (px, py, pz), components (1,2) on x,
+  // (0,2) on y, (0,1) on z, into words at f with tag.
+  auto send = [&](Word* f, const float* F, int px, int py, int pz, unsigned tag) {
+    for (int t = tid; t < 2 * vy * vz; t += kThreads) {
+      const int comp = t >= vy * vz, r = t - comp * vy * vz;
+      const int ly = r / vz, lz = r - ly * vz;
+      put(f + comp * fyz + ly * sz + lz, F[(1 + comp) * cells + (px * sy + ly) * sz + lz], tag);
     }
-    for (int t = tid; t < vx * vz; t += kThreads) {
-      const int lx = t / vz, lz = t - lx * vz;
-      const int c = (lx * sy + fy_) * sz + lz;
-      f[2 * fyz + lx * sz + lz] = F0[c];
-      f[2 * fyz + fxz + lx * sz + lz] = F2[c];
+    for (int t = tid; t < 2 * vx * vz; t += kThreads) {
+      const int comp = t >= vx * vz, r = t - comp * vx * vz;
+      const int lx = r / vz, lz = r - lx * vz;
+      put(f + 2 * fyz + comp * fxz + lx * sz + lz,
+          F[2 * comp * cells + (lx * sy + py) * sz + lz], tag);
     }
-    for (int t = tid; t < vx * vy; t += kThreads) {
-      const int lx = t / vy, ly = t - lx * vy;
-      const int c = (lx * sy + ly) * sz + fz_;
-      f[2 * fyz + 2 * fxz + lx * sy + ly] = F0[c];
-      f[2 * fyz + 2 * fxz + fxy + lx * sy + ly] = F1[c];
+    for (int t = tid; t < 2 * vx * vy; t += kThreads) {
+      const int comp = t >= vx * vy, r = t - comp * vx * vy;
+      const int lx = r / vy, ly = r - lx * vy;
+      put(f + 2 * fyz + 2 * fxz + comp * fxy + lx * sy + ly,
+          F[comp * cells + (lx * sy + ly) * sz + pz], tag);
     }
   };
-  float* myface = a.face + (size_t)wg * per;
-  write_faces(myface, H, vx - 1, vy - 1, vz - 1);
-  publish(flagH + wg, 1);
+  // Incoming faces (words at neighbour buffers + off) into FX, FY, FZ;
+  // false, uniformly, when a poll gave up.
+  auto receive = [&](int nxw, int nyw, int nzw, size_t off, unsigned tag) {
+    bool ok = true;
+    if (nxw >= 0) {
+      const Word* f = a.face + (size_t)nxw * per + off;
+      for (int t = tid; ok && t < 2 * vy * vz; t += kThreads) {
+        const int comp = t >= vy * vz, r = t - comp * vy * vz;
+        const int ly = r / vz, lz = r - ly * vz;
+        const int k = comp * fyz + ly * sz + lz;
+        ok = take(a, f + k, tag, FX[k]);
+      }
+    }
+    if (nyw >= 0) {
+      const Word* f = a.face + (size_t)nyw * per + off + 2 * fyz;
+      for (int t = tid; ok && t < 2 * vx * vz; t += kThreads) {
+        const int comp = t >= vx * vz, r = t - comp * vx * vz;
+        const int lx = r / vz, lz = r - lx * vz;
+        const int k = comp * fxz + lx * sz + lz;
+        ok = take(a, f + k, tag, FY[k]);
+      }
+    }
+    if (nzw >= 0) {
+      const Word* f = a.face + (size_t)nzw * per + off + 2 * fyz + 2 * fxz;
+      for (int t = tid; ok && t < 2 * vx * vy; t += kThreads) {
+        const int comp = t >= vx * vy, r = t - comp * vx * vy;
+        const int lx = r / vy, ly = r - lx * vy;
+        const int k = comp * fxy + lx * sy + ly;
+        ok = take(a, f + k, tag, FZ[k]);
+      }
+    }
+    if (!ok) bail = 1;
+    __syncthreads();
+    return bail == 0;
+  };
+
+  send(myface, H, vx - 1, vy - 1, vz - 1, 1u);
 
   for (int k = 0; k < nsteps; ++k) {
     const int t = t0 + k;
     // ------------------------------------------------------------ E half
-    if (!wait_for(a, flagH, wxl, wyl, wzl, k + 1, &bail)) return;
+    if (!receive(wxl, wyl, wzl, 0, 2u * k + 1u)) return;
     {
-      const float* fx = wxl >= 0 ? a.face + (size_t)wxl * per : nullptr;
-      const float* fy = wyl >= 0 ? a.face + (size_t)wyl * per : nullptr;
-      const float* fz = wzl >= 0 ? a.face + (size_t)wzl * per : nullptr;
       const bool src_here = a.source_x >= X0 && a.source_x < X0 + vx && t < a.steps;
       const float wave_y = src_here ? a.wave_h[2 * a.steps + t] : 0.f;
       const float wave_z = src_here ? a.wave_h[a.steps + t] : 0.f;
@@ -233,27 +277,27 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
         if (lx > 0) {
           my = H[cells + c - sy * sz];
           mz = H[2 * cells + c - sy * sz];
-        } else if (fx) {
-          my = fx[ly * sz + lz];
-          mz = fx[fyz + ly * sz + lz];
+        } else if (wxl >= 0) {
+          my = FX[ly * sz + lz];
+          mz = FX[fyz + ly * sz + lz];
         } else {
           my = mz = 0.f;
         }
         if (ly > 0) {
           yx = H[c - sz];
           yz = H[2 * cells + c - sz];
-        } else if (fy) {
-          yx = fy[2 * fyz + lx * sz + lz];
-          yz = fy[2 * fyz + fxz + lx * sz + lz];
+        } else if (wyl >= 0) {
+          yx = FY[lx * sz + lz];
+          yz = FY[fxz + lx * sz + lz];
         } else {
           yx = yz = 0.f;
         }
         if (lz > 0) {
           zx = H[c - 1];
           zy = H[cells + c - 1];
-        } else if (fz) {
-          zx = fz[2 * fyz + 2 * fxz + lx * sy + ly];
-          zy = fz[2 * fyz + 2 * fxz + fxy + lx * sy + ly];
+        } else if (wzl >= 0) {
+          zx = FZ[lx * sy + ly];
+          zy = FZ[fxy + lx * sy + ly];
         } else {
           zx = zy = 0.f;
         }
@@ -285,14 +329,10 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
       }
     }
     __syncthreads();
-    write_faces(myface + half, E, 0, 0, 0);  // low E faces for the low neighbours' H half
-    publish(flagE + wg, k + 1);
+    send(myface + half, E, 0, 0, 0, 2u * k + 2u);  // low E faces for the low neighbours
     // ------------------------------------------------------------ H half
-    if (!wait_for(a, flagE, wxh, wyh, wzh, k + 1, &bail)) return;
+    if (!receive(wxh, wyh, wzh, half, 2u * k + 2u)) return;
     {
-      const float* fx = wxh >= 0 ? a.face + (size_t)wxh * per + half : nullptr;
-      const float* fy = wyh >= 0 ? a.face + (size_t)wyh * per + half : nullptr;
-      const float* fz = wzh >= 0 ? a.face + (size_t)wzh * per + half : nullptr;
       const bool src_here = a.source_x >= X0 && a.source_x < X0 + vx && t < a.steps;
       const float wave_y = src_here ? a.wave_e[2 * a.steps + t] : 0.f;
       const float wave_z = src_here ? a.wave_e[a.steps + t] : 0.f;
@@ -309,27 +349,27 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
         if (lx + 1 < vx) {
           py = E[cells + c + sy * sz];
           pz = E[2 * cells + c + sy * sz];
-        } else if (fx) {
-          py = fx[ly * sz + lz];
-          pz = fx[fyz + ly * sz + lz];
+        } else if (wxh >= 0) {
+          py = FX[ly * sz + lz];
+          pz = FX[fyz + ly * sz + lz];
         } else {
           py = pz = 0.f;
         }
         if (ly + 1 < vy) {
           qx = E[c + sz];
           qz = E[2 * cells + c + sz];
-        } else if (fy) {
-          qx = fy[2 * fyz + lx * sz + lz];
-          qz = fy[2 * fyz + fxz + lx * sz + lz];
+        } else if (wyh >= 0) {
+          qx = FY[lx * sz + lz];
+          qz = FY[fxz + lx * sz + lz];
         } else {
           qx = qz = 0.f;
         }
         if (lz + 1 < vz) {
           rx = E[c + 1];
           ry = E[cells + c + 1];
-        } else if (fz) {
-          rx = fz[2 * fyz + 2 * fxz + lx * sy + ly];
-          ry = fz[2 * fyz + 2 * fxz + fxy + lx * sy + ly];
+        } else if (wzh >= 0) {
+          rx = FZ[lx * sy + ly];
+          ry = FZ[fxy + lx * sy + ly];
         } else {
           rx = ry = 0.f;
         }
@@ -378,8 +418,7 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
       }
     }
     __syncthreads();
-    write_faces(myface, H, vx - 1, vy - 1, vz - 1);
-    publish(flagH + wg, k + 2);
+    send(myface, H, vx - 1, vy - 1, vz - 1, 2u * k + 3u);
   }
 #pragma unroll 1
   for (int c = tid; c < cells; c += kThreads) {
@@ -395,9 +434,10 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
   }
 }
 
-__global__ void k_reset_flags(int* flags, int count) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
-    flags[i] = 0;
+__global__ void k_zero_words(Word* w, size_t count) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (size_t)gridDim.x * blockDim.x)
+    w[i] = 0;
 }
 
 __global__ void k_advance_counters(int* counters, int nsteps) {
@@ -410,7 +450,8 @@ struct Layout {
 };
 
 // Sub-box choice: the fewest cell passes per workgroup (ceil(cells / 1024)),
-// then the smaller sub-box surface (face traffic), then fewer workgroups.
+// then the smaller sub-box surface (face traffic), then fewer workgroups;
+// every candidate fits the LDS (lds_floats <= kLds).
 bool choose(int nx, int ny, int nz, int capacity, int want_sx, int want_sy, int want_sz,
             Layout& best) {
   bool found = false;
@@ -419,19 +460,19 @@ bool choose(int nx, int ny, int nz, int capacity, int want_sx, int want_sy, int 
     if (want_sz && sz != want_sz) continue;
     for (int sy = 1; sy <= ny; ++sy) {
       if (want_sy && sy != want_sy) continue;
-      if (sy * sz > kResidentCells) break;
-      const int top = min(nx, kResidentCells / (sy * sz));
+      if (7L * sy * sz > kLds) break;
+      const int top = min(nx, (int)(kLds / (7L * sy * sz)));
       for (int sx = want_sx ? want_sx : 1; sx <= (want_sx ? want_sx : top); ++sx) {
-        if (sx * sy * sz > kResidentCells) break;
         // even out the cells per workgroup along each axis without adding
         // workgroups: an evened size never exceeds the first one
         int gx = (nx + sx - 1) / sx, gy = (ny + sy - 1) / sy, gz = (nz + sz - 1) / sz;
         const int ex = (nx + gx - 1) / gx, ey = (ny + gy - 1) / gy, ez = (nz + gz - 1) / gz;
+        if (lds_floats(ex, ey, ez) > kLds) continue;
         gx = (nx + ex - 1) / ex;
         gy = (ny + ey - 1) / ey;
         gz = (nz + ez - 1) / ez;
         const long blocks = (long)gx * gy * gz;
-        if (blocks > capacity || ex + ey + ez > kCoeffRows) continue;
+        if (blocks > capacity) continue;
         const long key[3] = {((long)ex * ey * ez + kThreads - 1) / kThreads,
                              (long)ex * ey + (long)ex * ez + (long)ey * ez, blocks};
         const bool better = !found || key[0] < best_key[0] ||
@@ -447,6 +488,10 @@ bool choose(int nx, int ny, int nz, int capacity, int want_sx, int want_sy, int 
     }
   }
   return found;
+}
+
+int64_t workspace_bytes(int sx, int sy, int sz, int blocks) {
+  return (int64_t)(face_words(sx, sy, sz) * blocks * sizeof(Word)) + 64 * (int64_t)sizeof(int);
 }
 
 }  // namespace
@@ -479,8 +524,8 @@ int wfd_resident_layout_for(const wfd_yee_plan* p, int sx, int sy, int sz,
   Layout c;
   if (!choose(p->nx, p->ny, p->nz, capacity, sx, sy, sz, c)) {
     *out = L;
-    set_error("resident_layout: %dx%dx%d does not fit %d co-resident workgroups of %d cells",
-              p->nx, p->ny, p->nz, capacity, kResidentCells);
+    set_error("resident_layout: %dx%dx%d does not fit %d co-resident workgroups of %d LDS floats",
+              p->nx, p->ny, p->nz, capacity, kLds);
     return WFD_ERR_UNSUPPORTED;
   }
   L.sx = c.sx;
@@ -490,9 +535,7 @@ int wfd_resident_layout_for(const wfd_yee_plan* p, int sx, int sy, int sz,
   L.gy = c.gy;
   L.gz = c.gz;
   L.blocks = c.blocks;
-  const size_t per = 4 * (size_t)(c.sy * c.sz + c.sx * c.sz + c.sx * c.sy);
-  L.workspace_bytes =
-      (int64_t)(per * c.blocks * sizeof(float) + (2 * (size_t)c.blocks + 64) * sizeof(int));
+  L.workspace_bytes = workspace_bytes(c.sx, c.sy, c.sz, c.blocks);
   *out = L;
   return WFD_OK;
 }
@@ -500,10 +543,12 @@ int wfd_resident_layout_for(const wfd_yee_plan* p, int sx, int sy, int sz,
 int wfd_resident_run(const wfd_yee_plan* p, const wfd_resident_layout* L, void* workspace,
                      int nsteps, int spin_limit, void* stream) {
   WFD_REQUIRE(p && L && workspace, "resident_run: null argument");
-  WFD_REQUIRE(nsteps >= 0, "resident_run: negative step count");
+  WFD_REQUIRE(nsteps >= 0 && nsteps < (1 << 30), "resident_run: step count out of range");
   WFD_REQUIRE(p->e && p->h && p->inv_eps && p->psi_e && p->psi_h && p->coeff_e && p->coeff_h &&
                   p->counters,
               "resident_run: null field, psi, coeff or counter pointer");
+  WFD_REQUIRE(p->nx > 0 && p->ny > 0 && p->nz > 0 && cells(p->nx, p->ny, p->nz) <= kMaxCells,
+              "resident_run: bad grid");
   WFD_REQUIRE(p->coeff_rows >= p->nx && p->coeff_rows >= p->ny && p->coeff_rows >= p->nz,
               "resident_run: coeff_rows < max(shape)");
   WFD_REQUIRE(p->source_x < p->nx, "resident_run: source plane outside the grid");
@@ -513,17 +558,16 @@ int wfd_resident_run(const wfd_yee_plan* p, const wfd_resident_layout* L, void* 
   WFD_REQUIRE(p->nstage >= 0 && p->nstage <= WFD_MAX_STAGES, "resident_run: 0..%d stages",
               WFD_MAX_STAGES);
   WFD_REQUIRE(p->nstage == 0 || p->slots > 0, "resident_run: stage slots must be positive");
-  WFD_REQUIRE(L->sx > 0 && L->sy > 0 && L->sz > 0 && L->sx * L->sy * L->sz <= kResidentCells &&
-                  L->sx + L->sy + L->sz <= kCoeffRows &&
+  WFD_REQUIRE(L->sx > 0 && L->sy > 0 && L->sz > 0 && lds_floats(L->sx, L->sy, L->sz) <= kLds &&
                   L->gx == (p->nx + L->sx - 1) / L->sx &&
                   L->gy == (p->ny + L->sy - 1) / L->sy &&
                   L->gz == (p->nz + L->sz - 1) / L->sz && L->blocks == L->gx * L->gy * L->gz &&
                   L->blocks <= L->capacity,
               "resident_run: layout does not match the plan");
-  const size_t per = 4 * (size_t)(L->sy * L->sz + L->sx * L->sz + L->sx * L->sy);
-  WFD_REQUIRE(L->workspace_bytes ==
-                  (int64_t)(per * L->blocks * sizeof(float) + (2 * (size_t)L->blocks + 64) * sizeof(int)),
+  WFD_REQUIRE(L->workspace_bytes == workspace_bytes(L->sx, L->sy, L->sz, L->blocks),
               "resident_run: workspace size does not match the layout");
+  WFD_REQUIRE((reinterpret_cast<uintptr_t>(workspace) & 7) == 0,
+              "resident_run: workspace must be 8-byte aligned");
   if (nsteps == 0) return WFD_OK;
   ResArgs a;
   a.e = p->e;
@@ -552,18 +596,19 @@ int wfd_resident_run(const wfd_yee_plan* p, const wfd_resident_layout* L, void* 
   a.gx = L->gx;
   a.gy = L->gy;
   a.gz = L->gz;
-  a.face = reinterpret_cast<float*>(workspace);
-  a.flags = reinterpret_cast<int*>(a.face + per * L->blocks);
-  a.error = a.flags + 2 * L->blocks;
+  const size_t words = face_words(L->sx, L->sy, L->sz) * L->blocks;
+  a.face = reinterpret_cast<Word*>(workspace);
+  a.error = reinterpret_cast<int*>(a.face + words);
   a.spin_limit = spin_limit > 0 ? spin_limit : (1 << 22);
   a.nstage = p->nstage;
   a.slots = p->slots > 0 ? p->slots : 1;
+  for (int b = 0; b < WFD_MAX_STAGES; ++b) a.stages[b] = StageBox{};
   for (int b = 0; b < p->nstage; ++b)
     WFD_REQUIRE(to_stage_box(p->stages[b], p->nx, p->ny, p->nz, a.stages[b]) == WFD_OK,
                 "resident_run: stage %d outside the grid or without a buffer", b);
   hipStream_t s = as_stream(stream);
-  hipLaunchKernelGGL(k_reset_flags, dim3(8), dim3(256), 0, s, a.flags, 2 * L->blocks);
-  WFD_LAUNCH_CHECK("resident_run(reset flags)");
+  hipLaunchKernelGGL(k_zero_words, dim3(256), dim3(256), 0, s, a.face, words);
+  WFD_LAUNCH_CHECK("resident_run(zero faces)");
   void* args[] = {&a, &nsteps};
   WFD_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_resident),
                                      dim3(L->blocks), dim3(kThreads), args, 0, s));

@@ -406,9 +406,9 @@ class Solver:
         self.resident = True
 
     def _error_word(self):
+        """Index (int32) of the error word: after the 64-bit face words."""
         L = self.layout
-        face = 4 * (L.sy * L.sz + L.sx * L.sz + L.sx * L.sy) * L.blocks
-        return face + 2 * L.blocks
+        return 2 * 4 * (L.sy * L.sz + L.sx * L.sz + L.sx * L.sy) * L.blocks
 
     def check(self):
         """Raise if a resident run timed out waiting for a neighbouring
