@@ -35,6 +35,7 @@ namespace {
 
 constexpr int kThreads = 1024;
 constexpr int kLds = 40896;  // floats of LDS per workgroup (163,584 bytes)
+constexpr int kBatch = 4;    // face words a thread polls per round trip
 
 typedef unsigned long long Word;
 
@@ -220,34 +221,59 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
     }
   };
   // Incoming faces (words at neighbour buffers + off) into FX, FY, FZ;
-  // false, uniformly, when a poll gave up.
+  // false, uniformly, when a poll gave up.  The words of the three faces form
+  // one index space; each thread issues up to kBatch loads before it checks
+  // any, so the polls of a half-step cost one round trip, not three.
   auto receive = [&](int nxw, int nyw, int nzw, size_t off, unsigned tag) {
-    bool ok = true;
-    if (nxw >= 0) {
-      const Word* f = a.face + (size_t)nxw * per + off;
-      for (int t = tid; ok && t < 2 * vy * vz; t += kThreads) {
-        const int comp = t >= vy * vz, r = t - comp * vy * vz;
+    const int cx_ = nxw >= 0 ? 2 * vy * vz : 0;
+    const int cy_ = nyw >= 0 ? 2 * vx * vz : 0;
+    const int cz_ = nzw >= 0 ? 2 * vx * vy : 0;
+    const int total = cx_ + cy_ + cz_;
+    auto locate = [&](int u, const Word*& src, float*& dst) {
+      if (u < cx_) {
+        const int comp = u >= vy * vz, r = u - comp * vy * vz;
         const int ly = r / vz, lz = r - ly * vz;
         const int k = comp * fyz + ly * sz + lz;
-        ok = take(a, f + k, tag, FX[k]);
-      }
-    }
-    if (nyw >= 0) {
-      const Word* f = a.face + (size_t)nyw * per + off + 2 * fyz;
-      for (int t = tid; ok && t < 2 * vx * vz; t += kThreads) {
-        const int comp = t >= vx * vz, r = t - comp * vx * vz;
+        src = a.face + (size_t)nxw * per + off + k;
+        dst = FX + k;
+      } else if (u < cx_ + cy_) {
+        u -= cx_;
+        const int comp = u >= vx * vz, r = u - comp * vx * vz;
         const int lx = r / vz, lz = r - lx * vz;
         const int k = comp * fxz + lx * sz + lz;
-        ok = take(a, f + k, tag, FY[k]);
-      }
-    }
-    if (nzw >= 0) {
-      const Word* f = a.face + (size_t)nzw * per + off + 2 * fyz + 2 * fxz;
-      for (int t = tid; ok && t < 2 * vx * vy; t += kThreads) {
-        const int comp = t >= vx * vy, r = t - comp * vx * vy;
+        src = a.face + (size_t)nyw * per + off + 2 * fyz + k;
+        dst = FY + k;
+      } else {
+        u -= cx_ + cy_;
+        const int comp = u >= vx * vy, r = u - comp * vx * vy;
         const int lx = r / vy, ly = r - lx * vy;
         const int k = comp * fxy + lx * sy + ly;
-        ok = take(a, f + k, tag, FZ[k]);
+        src = a.face + (size_t)nzw * per + off + 2 * fyz + 2 * fxz + k;
+        dst = FZ + k;
+      }
+    };
+    bool ok = true;
+    for (int base = 0; ok && base < total; base += kBatch * kThreads) {
+      const Word* src[kBatch];
+      float* dst[kBatch];
+      Word w[kBatch];
+#pragma unroll
+      for (int q = 0; q < kBatch; ++q) {
+        const int u = base + q * kThreads + tid;
+        src[q] = nullptr;
+        if (u < total) {
+          locate(u, src[q], dst[q]);
+          w[q] = __hip_atomic_load(src[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kBatch; ++q) {
+        if (!src[q] || !ok) continue;
+        if ((unsigned)(w[q] >> 32) == tag) {
+          *dst[q] = __uint_as_float((unsigned)w[q]);
+        } else {
+          ok = take(a, src[q], tag, *dst[q]);
+        }
       }
     }
     if (!ok) bail = 1;
@@ -265,71 +291,79 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
       const bool src_here = a.source_x >= X0 && a.source_x < X0 + vx && t < a.steps;
       const float wave_y = src_here ? a.wave_h[2 * a.steps + t] : 0.f;
       const float wave_z = src_here ? a.wave_h[a.steps + t] : 0.f;
+      // boundary cells (the low faces the neighbours wait for) first, then
+      // the interior while the faces travel
+      auto pass = [&](bool boundary) {
 #pragma unroll 1
-      for (int c = tid; c < cells; c += kThreads) {
-        int lx, ly, lz;
-        coords(c, lx, ly, lz);
-        if (lx >= vx || ly >= vy || lz >= vz) continue;
-        const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
-        const int i = (x * ny + y) * nz + z;
-        const float hx = H[c], hy = H[cells + c], hz = H[2 * cells + c];
-        float my, mz, yx, yz, zx, zy;
-        if (lx > 0) {
-          my = H[cells + c - sy * sz];
-          mz = H[2 * cells + c - sy * sz];
-        } else if (wxl >= 0) {
-          my = FX[ly * sz + lz];
-          mz = FX[fyz + ly * sz + lz];
-        } else {
-          my = mz = 0.f;
+        for (int c = tid; c < cells; c += kThreads) {
+          int lx, ly, lz;
+          coords(c, lx, ly, lz);
+          if (lx >= vx || ly >= vy || lz >= vz) continue;
+          if ((lx == 0 || ly == 0 || lz == 0) != boundary) continue;
+          const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
+          const int i = (x * ny + y) * nz + z;
+          const float hx = H[c], hy = H[cells + c], hz = H[2 * cells + c];
+          float my, mz, yx, yz, zx, zy;
+          if (lx > 0) {
+            my = H[cells + c - sy * sz];
+            mz = H[2 * cells + c - sy * sz];
+          } else if (wxl >= 0) {
+            my = FX[ly * sz + lz];
+            mz = FX[fyz + ly * sz + lz];
+          } else {
+            my = mz = 0.f;
+          }
+          if (ly > 0) {
+            yx = H[c - sz];
+            yz = H[2 * cells + c - sz];
+          } else if (wyl >= 0) {
+            yx = FY[lx * sz + lz];
+            yz = FY[fxz + lx * sz + lz];
+          } else {
+            yx = yz = 0.f;
+          }
+          if (lz > 0) {
+            zx = H[c - 1];
+            zy = H[cells + c - 1];
+          } else if (wzl >= 0) {
+            zx = FZ[lx * sy + ly];
+            zy = FZ[fxy + lx * sy + ly];
+          } else {
+            zx = zy = 0.f;
+          }
+          const float dx1 = hy - my, dx2 = hz - mz;
+          const float dy0 = hx - yx, dy2 = hz - yz;
+          const float dz0 = hx - zx, dz1 = hy - zy;
+          const float* cx = cE + lx * 3;
+          const float* cy = cE + (sx + ly) * 3;
+          const float* cz = cE + (sx + sy + lz) * 3;
+          float* psi = a.psi_e + i;
+          const float dxz = cpml_step(dx2, cx[0], cx[1], cx[2], psi);
+          const float dxy = cpml_step(dx1, cx[0], cx[1], cx[2], psi + n);
+          const float dyx = cpml_step(dy0, cy[0], cy[1], cy[2], psi + 2 * n);
+          const float dyz = cpml_step(dy2, cy[0], cy[1], cy[2], psi + 3 * n);
+          const float dzy = cpml_step(dz1, cz[0], cz[1], cz[2], psi + 4 * n);
+          const float dzx = cpml_step(dz0, cz[0], cz[1], cz[2], psi + 5 * n);
+          const float ie = IE[c];
+          float ex = E[c] + (S * (dyz - dzy)) * ie;
+          float ey = E[cells + c] + (S * (dzx - dxz)) * ie;
+          float ez = E[2 * cells + c] + (S * (dxy - dyx)) * ie;
+          if (src_here && x == a.source_x) {
+            const int o = y * nz + z;
+            ey += ((a.src_h[2 * P + o] * wave_y) * S) * ie;
+            ez -= ((a.src_h[P + o] * wave_z) * S) * ie;
+          }
+          E[c] = ex;
+          E[cells + c] = ey;
+          E[2 * cells + c] = ez;
         }
-        if (ly > 0) {
-          yx = H[c - sz];
-          yz = H[2 * cells + c - sz];
-        } else if (wyl >= 0) {
-          yx = FY[lx * sz + lz];
-          yz = FY[fxz + lx * sz + lz];
-        } else {
-          yx = yz = 0.f;
-        }
-        if (lz > 0) {
-          zx = H[c - 1];
-          zy = H[cells + c - 1];
-        } else if (wzl >= 0) {
-          zx = FZ[lx * sy + ly];
-          zy = FZ[fxy + lx * sy + ly];
-        } else {
-          zx = zy = 0.f;
-        }
-        const float dx1 = hy - my, dx2 = hz - mz;
-        const float dy0 = hx - yx, dy2 = hz - yz;
-        const float dz0 = hx - zx, dz1 = hy - zy;
-        const float* cx = cE + lx * 3;
-        const float* cy = cE + (sx + ly) * 3;
-        const float* cz = cE + (sx + sy + lz) * 3;
-        float* psi = a.psi_e + i;
-        const float dxz = cpml_step(dx2, cx[0], cx[1], cx[2], psi);
-        const float dxy = cpml_step(dx1, cx[0], cx[1], cx[2], psi + n);
-        const float dyx = cpml_step(dy0, cy[0], cy[1], cy[2], psi + 2 * n);
-        const float dyz = cpml_step(dy2, cy[0], cy[1], cy[2], psi + 3 * n);
-        const float dzy = cpml_step(dz1, cz[0], cz[1], cz[2], psi + 4 * n);
-        const float dzx = cpml_step(dz0, cz[0], cz[1], cz[2], psi + 5 * n);
-        const float ie = IE[c];
-        float ex = E[c] + (S * (dyz - dzy)) * ie;
-        float ey = E[cells + c] + (S * (dzx - dxz)) * ie;
-        float ez = E[2 * cells + c] + (S * (dxy - dyx)) * ie;
-        if (src_here && x == a.source_x) {
-          const int o = y * nz + z;
-          ey += ((a.src_h[2 * P + o] * wave_y) * S) * ie;
-          ez -= ((a.src_h[P + o] * wave_z) * S) * ie;
-        }
-        E[c] = ex;
-        E[cells + c] = ey;
-        E[2 * cells + c] = ez;
-      }
+      };
+      pass(true);
+      __syncthreads();
+      send(myface + half, E, 0, 0, 0, 2u * k + 2u);  // low E faces for the low neighbours
+      pass(false);
     }
     __syncthreads();
-    send(myface + half, E, 0, 0, 0, 2u * k + 2u);  // low E faces for the low neighbours
     // ------------------------------------------------------------ H half
     if (!receive(wxh, wyh, wzh, half, 2u * k + 2u)) return;
     {
@@ -337,88 +371,94 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
       const float wave_y = src_here ? a.wave_e[2 * a.steps + t] : 0.f;
       const float wave_z = src_here ? a.wave_e[a.steps + t] : 0.f;
       const int slot = a.nstage ? t % a.slots : 0;
+      auto pass = [&](bool boundary) {
 #pragma unroll 1
-      for (int c = tid; c < cells; c += kThreads) {
-        int lx, ly, lz;
-        coords(c, lx, ly, lz);
-        if (lx >= vx || ly >= vy || lz >= vz) continue;
-        const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
-        const int i = (x * ny + y) * nz + z;
-        const float ex = E[c], ey = E[cells + c], ez = E[2 * cells + c];
-        float py, pz, qx, qz, rx, ry;
-        if (lx + 1 < vx) {
-          py = E[cells + c + sy * sz];
-          pz = E[2 * cells + c + sy * sz];
-        } else if (wxh >= 0) {
-          py = FX[ly * sz + lz];
-          pz = FX[fyz + ly * sz + lz];
-        } else {
-          py = pz = 0.f;
-        }
-        if (ly + 1 < vy) {
-          qx = E[c + sz];
-          qz = E[2 * cells + c + sz];
-        } else if (wyh >= 0) {
-          qx = FY[lx * sz + lz];
-          qz = FY[fxz + lx * sz + lz];
-        } else {
-          qx = qz = 0.f;
-        }
-        if (lz + 1 < vz) {
-          rx = E[c + 1];
-          ry = E[cells + c + 1];
-        } else if (wzh >= 0) {
-          rx = FZ[lx * sy + ly];
-          ry = FZ[fxy + lx * sy + ly];
-        } else {
-          rx = ry = 0.f;
-        }
-        const float dx1 = py - ey, dx2 = pz - ez;
-        const float dy0 = qx - ex, dy2 = qz - ez;
-        const float dz0 = rx - ex, dz1 = ry - ey;
-        const float* cx = cH + lx * 3;
-        const float* cy = cH + (sx + ly) * 3;
-        const float* cz = cH + (sx + sy + lz) * 3;
-        float* psi = a.psi_h + i;
-        const float dxz = cpml_step(dx2, cx[0], cx[1], cx[2], psi);
-        const float dxy = cpml_step(dx1, cx[0], cx[1], cx[2], psi + n);
-        const float dyx = cpml_step(dy0, cy[0], cy[1], cy[2], psi + 2 * n);
-        const float dyz = cpml_step(dy2, cy[0], cy[1], cy[2], psi + 3 * n);
-        const float dzy = cpml_step(dz1, cz[0], cz[1], cz[2], psi + 4 * n);
-        const float dzx = cpml_step(dz0, cz[0], cz[1], cz[2], psi + 5 * n);
-        const float ox = H[c], oy = H[cells + c], oz = H[2 * cells + c];
-        float hx = ox - S * (dyz - dzy);
-        float hy = oy - S * (dzx - dxz);
-        float hz = oz - S * (dxy - dyx);
-        if (src_here && x == a.source_x) {
-          const int o = y * nz + z;
-          hy -= (a.src_e[2 * P + o] * wave_y) * S;
-          hz += (a.src_e[P + o] * wave_z) * S;
-        }
-        H[c] = hx;
-        H[cells + c] = hy;
-        H[2 * cells + c] = hz;
-        for (int b = 0; b < a.nstage; ++b) {
-          const StageBox& s = a.stages[b];
-          if (x >= s.x0 && x < s.x1 && y >= s.y0 && y < s.y1 && z >= s.z0 && z < s.z1) {
-            const int cnt = s.count;
-            float* d = s.buf + (size_t)slot * 9 * cnt +
-                       ((x - s.x0) * (s.y1 - s.y0) + (y - s.y0)) * (s.z1 - s.z0) + (z - s.z0);
-            d[0] = ex;
-            d[cnt] = ey;
-            d[2 * cnt] = ez;
-            d[3 * cnt] = hx;
-            d[4 * cnt] = hy;
-            d[5 * cnt] = hz;
-            d[6 * cnt] = ox;
-            d[7 * cnt] = oy;
-            d[8 * cnt] = oz;
+        for (int c = tid; c < cells; c += kThreads) {
+          int lx, ly, lz;
+          coords(c, lx, ly, lz);
+          if (lx >= vx || ly >= vy || lz >= vz) continue;
+          if ((lx == vx - 1 || ly == vy - 1 || lz == vz - 1) != boundary) continue;
+          const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
+          const int i = (x * ny + y) * nz + z;
+          const float ex = E[c], ey = E[cells + c], ez = E[2 * cells + c];
+          float py, pz, qx, qz, rx, ry;
+          if (lx + 1 < vx) {
+            py = E[cells + c + sy * sz];
+            pz = E[2 * cells + c + sy * sz];
+          } else if (wxh >= 0) {
+            py = FX[ly * sz + lz];
+            pz = FX[fyz + ly * sz + lz];
+          } else {
+            py = pz = 0.f;
+          }
+          if (ly + 1 < vy) {
+            qx = E[c + sz];
+            qz = E[2 * cells + c + sz];
+          } else if (wyh >= 0) {
+            qx = FY[lx * sz + lz];
+            qz = FY[fxz + lx * sz + lz];
+          } else {
+            qx = qz = 0.f;
+          }
+          if (lz + 1 < vz) {
+            rx = E[c + 1];
+            ry = E[cells + c + 1];
+          } else if (wzh >= 0) {
+            rx = FZ[lx * sy + ly];
+            ry = FZ[fxy + lx * sy + ly];
+          } else {
+            rx = ry = 0.f;
+          }
+          const float dx1 = py - ey, dx2 = pz - ez;
+          const float dy0 = qx - ex, dy2 = qz - ez;
+          const float dz0 = rx - ex, dz1 = ry - ey;
+          const float* cx = cH + lx * 3;
+          const float* cy = cH + (sx + ly) * 3;
+          const float* cz = cH + (sx + sy + lz) * 3;
+          float* psi = a.psi_h + i;
+          const float dxz = cpml_step(dx2, cx[0], cx[1], cx[2], psi);
+          const float dxy = cpml_step(dx1, cx[0], cx[1], cx[2], psi + n);
+          const float dyx = cpml_step(dy0, cy[0], cy[1], cy[2], psi + 2 * n);
+          const float dyz = cpml_step(dy2, cy[0], cy[1], cy[2], psi + 3 * n);
+          const float dzy = cpml_step(dz1, cz[0], cz[1], cz[2], psi + 4 * n);
+          const float dzx = cpml_step(dz0, cz[0], cz[1], cz[2], psi + 5 * n);
+          const float ox = H[c], oy = H[cells + c], oz = H[2 * cells + c];
+          float hx = ox - S * (dyz - dzy);
+          float hy = oy - S * (dzx - dxz);
+          float hz = oz - S * (dxy - dyx);
+          if (src_here && x == a.source_x) {
+            const int o = y * nz + z;
+            hy -= (a.src_e[2 * P + o] * wave_y) * S;
+            hz += (a.src_e[P + o] * wave_z) * S;
+          }
+          H[c] = hx;
+          H[cells + c] = hy;
+          H[2 * cells + c] = hz;
+          for (int b = 0; b < a.nstage; ++b) {
+            const StageBox& s = a.stages[b];
+            if (x >= s.x0 && x < s.x1 && y >= s.y0 && y < s.y1 && z >= s.z0 && z < s.z1) {
+              const int cnt = s.count;
+              float* d = s.buf + (size_t)slot * 9 * cnt +
+                         ((x - s.x0) * (s.y1 - s.y0) + (y - s.y0)) * (s.z1 - s.z0) + (z - s.z0);
+              d[0] = ex;
+              d[cnt] = ey;
+              d[2 * cnt] = ez;
+              d[3 * cnt] = hx;
+              d[4 * cnt] = hy;
+              d[5 * cnt] = hz;
+              d[6 * cnt] = ox;
+              d[7 * cnt] = oy;
+              d[8 * cnt] = oz;
+            }
           }
         }
-      }
+      };
+      pass(true);
+      __syncthreads();
+      send(myface, H, vx - 1, vy - 1, vz - 1, 2u * k + 3u);  // high H faces
+      pass(false);
     }
     __syncthreads();
-    send(myface, H, vx - 1, vy - 1, vz - 1, 2u * k + 3u);
   }
 #pragma unroll 1
   for (int c = tid; c < cells; c += kThreads) {

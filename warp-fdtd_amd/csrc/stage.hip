@@ -167,9 +167,11 @@ __global__ __launch_bounds__(kBlock) void k_replay(ReplayArgs args, int slots,
   double re[kFreqGroup], im[kFreqGroup];
 #pragma unroll
   for (int q = 0; q < kFreqGroup; ++q) re[q] = im[q] = 0.0;
-  for (int j = 0; j < m; ++j) {
-    const int t = last - m + 1 + j;
-    if (t < 0 || t >= r.steps) continue;
+  // steps t in [t_lo, t_hi]: the chunk's steps inside the recording window;
+  // unrolled by 4 so the colocation loads of four steps are in flight at once
+  const int t_lo = max(last - m + 1, 0), t_hi = min(last, r.steps - 1);
+#pragma unroll 4
+  for (int t = t_lo; t <= t_hi; ++t) {
     s.base = r.stage + (size_t)(t % slots) * 9 * r.fcount;
     const double v = (double)colocated(s, c, x, y, z, r.exact);
     const double* w = r.weights64 + 2 * ((size_t)t * r.nf + f0);
