@@ -100,7 +100,10 @@ __device__ __forceinline__ bool take(const ResArgs& a, const Word* p, unsigned t
     if ((spins & 63) == 0 &&
         __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
       return false;
-    __builtin_amdgcn_s_sleep(1);
+    if (spins < 8)
+      __builtin_amdgcn_s_sleep(2);
+    else
+      __builtin_amdgcn_s_sleep(8);
   }
 }
 
@@ -253,6 +256,18 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
       }
     };
     bool ok = true;
+    // one lane per wavefront waits for the wave's first word (the other 63
+    // lanes are masked off meanwhile), so a waiting workgroup keeps 16 polls
+    // in flight instead of 1024
+    {
+      const int u0 = tid & ~63;
+      if ((tid & 63) == 0 && u0 < total) {
+        const Word* src0;
+        float* dst0;
+        locate(u0, src0, dst0);
+        ok = take(a, src0, tag, *dst0);
+      }
+    }
     for (int base = 0; ok && base < total; base += kBatch * kThreads) {
       const Word* src[kBatch];
       float* dst[kBatch];
