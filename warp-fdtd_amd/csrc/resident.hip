@@ -107,10 +107,11 @@ __device__ __forceinline__ bool take(const ResArgs& a, const Word* p, unsigned t
   }
 }
 
-__device__ __forceinline__ float cpml_step(float d, float a, float b, float k,
-                                           float* __restrict__ psi_at) {
+// One CPML partial with the psi value already loaded (kernels.py:29-47).
+__device__ __forceinline__ float cpml_pre(float d, float a, float b, float k, float psi_old,
+                                          float* __restrict__ psi_at) {
   if (a != 0.f) {
-    const float p = b * (*psi_at) + a * d;
+    const float p = b * psi_old + a * d;
     *psi_at = p;
     return k * d + p;
   }
@@ -298,182 +299,215 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
 
   send(myface, H, vx - 1, vy - 1, vz - 1, 1u);
 
+  // Compute: thread t < sy*sz owns the column (ly, lz) = (t / sz, t % sz) of
+  // the sub-box and marches lx upward, so the x neighbour comes from the
+  // previous (E half) or next (H half) plane in registers, the y and z CPML
+  // rows are per-thread constants, and the psi values of the next plane are
+  // loaded while the current one is updated.
+  const int ly = tid / sz, lz = tid - (tid / sz) * sz;
+  const bool column = tid < sy * sz && ly < vy && lz < vz;
+  const int y = Y0 + ly, z = Z0 + lz;
+  const int col = ly * sz + lz;  // cell index of the column at lx = 0
+  const int plane = sy * sz;      // cell stride of lx
+  const int o = y * nz + z;       // source-plane offset
+  float ya_e = 0.f, yb_e = 0.f, yk_e = 0.f, za_e = 0.f, zb_e = 0.f, zk_e = 0.f;
+  float ya_h = 0.f, yb_h = 0.f, yk_h = 0.f, za_h = 0.f, zb_h = 0.f, zk_h = 0.f;
+  if (column) {
+    ya_e = cE[(sx + ly) * 3];
+    yb_e = cE[(sx + ly) * 3 + 1];
+    yk_e = cE[(sx + ly) * 3 + 2];
+    za_e = cE[(sx + sy + lz) * 3];
+    zb_e = cE[(sx + sy + lz) * 3 + 1];
+    zk_e = cE[(sx + sy + lz) * 3 + 2];
+    ya_h = cH[(sx + ly) * 3];
+    yb_h = cH[(sx + ly) * 3 + 1];
+    yk_h = cH[(sx + ly) * 3 + 2];
+    za_h = cH[(sx + sy + lz) * 3];
+    zb_h = cH[(sx + sy + lz) * 3 + 1];
+    zk_h = cH[(sx + sy + lz) * 3 + 2];
+  }
+  // psi of plane lx (six slots, zero where the axis has no CPML there)
+  auto load_psi = [&](const float* psi, const float* cf, float ya, float za, int lx,
+                      float (&p)[6]) {
+    const int i = (X0 + lx) * P + o;
+    const bool ax = cf[lx * 3] != 0.f;
+    p[0] = ax ? psi[i] : 0.f;
+    p[1] = ax ? psi[n + i] : 0.f;
+    p[2] = ya != 0.f ? psi[2 * n + i] : 0.f;
+    p[3] = ya != 0.f ? psi[3 * n + i] : 0.f;
+    p[4] = za != 0.f ? psi[4 * n + i] : 0.f;
+    p[5] = za != 0.f ? psi[5 * n + i] : 0.f;
+  };
+
   for (int k = 0; k < nsteps; ++k) {
     const int t = t0 + k;
     // ------------------------------------------------------------ E half
     if (!receive(wxl, wyl, wzl, 0, 2u * k + 1u)) return;
-    {
+    if (column) {
       const bool src_here = a.source_x >= X0 && a.source_x < X0 + vx && t < a.steps;
       const float wave_y = src_here ? a.wave_h[2 * a.steps + t] : 0.f;
       const float wave_z = src_here ? a.wave_h[a.steps + t] : 0.f;
-      // boundary cells (the low faces the neighbours wait for) first, then
-      // the interior while the faces travel
-      auto pass = [&](bool boundary) {
+      float my, mz;  // H_y, H_z of plane lx - 1
+      if (wxl >= 0) {
+        my = FX[ly * sz + lz];
+        mz = FX[fyz + ly * sz + lz];
+      } else {
+        my = mz = 0.f;
+      }
+      float cur[6], nxt[6];
+      load_psi(a.psi_e, cE, ya_e, za_e, 0, cur);
 #pragma unroll 1
-        for (int c = tid; c < cells; c += kThreads) {
-          int lx, ly, lz;
-          coords(c, lx, ly, lz);
-          if (lx >= vx || ly >= vy || lz >= vz) continue;
-          if ((lx == 0 || ly == 0 || lz == 0) != boundary) continue;
-          const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
-          const int i = (x * ny + y) * nz + z;
-          const float hx = H[c], hy = H[cells + c], hz = H[2 * cells + c];
-          float my, mz, yx, yz, zx, zy;
-          if (lx > 0) {
-            my = H[cells + c - sy * sz];
-            mz = H[2 * cells + c - sy * sz];
-          } else if (wxl >= 0) {
-            my = FX[ly * sz + lz];
-            mz = FX[fyz + ly * sz + lz];
-          } else {
-            my = mz = 0.f;
-          }
-          if (ly > 0) {
-            yx = H[c - sz];
-            yz = H[2 * cells + c - sz];
-          } else if (wyl >= 0) {
-            yx = FY[lx * sz + lz];
-            yz = FY[fxz + lx * sz + lz];
-          } else {
-            yx = yz = 0.f;
-          }
-          if (lz > 0) {
-            zx = H[c - 1];
-            zy = H[cells + c - 1];
-          } else if (wzl >= 0) {
-            zx = FZ[lx * sy + ly];
-            zy = FZ[fxy + lx * sy + ly];
-          } else {
-            zx = zy = 0.f;
-          }
-          const float dx1 = hy - my, dx2 = hz - mz;
-          const float dy0 = hx - yx, dy2 = hz - yz;
-          const float dz0 = hx - zx, dz1 = hy - zy;
-          const float* cx = cE + lx * 3;
-          const float* cy = cE + (sx + ly) * 3;
-          const float* cz = cE + (sx + sy + lz) * 3;
-          float* psi = a.psi_e + i;
-          const float dxz = cpml_step(dx2, cx[0], cx[1], cx[2], psi);
-          const float dxy = cpml_step(dx1, cx[0], cx[1], cx[2], psi + n);
-          const float dyx = cpml_step(dy0, cy[0], cy[1], cy[2], psi + 2 * n);
-          const float dyz = cpml_step(dy2, cy[0], cy[1], cy[2], psi + 3 * n);
-          const float dzy = cpml_step(dz1, cz[0], cz[1], cz[2], psi + 4 * n);
-          const float dzx = cpml_step(dz0, cz[0], cz[1], cz[2], psi + 5 * n);
-          const float ie = IE[c];
-          float ex = E[c] + (S * (dyz - dzy)) * ie;
-          float ey = E[cells + c] + (S * (dzx - dxz)) * ie;
-          float ez = E[2 * cells + c] + (S * (dxy - dyx)) * ie;
-          if (src_here && x == a.source_x) {
-            const int o = y * nz + z;
-            ey += ((a.src_h[2 * P + o] * wave_y) * S) * ie;
-            ez -= ((a.src_h[P + o] * wave_z) * S) * ie;
-          }
-          E[c] = ex;
-          E[cells + c] = ey;
-          E[2 * cells + c] = ez;
+      for (int lx = 0; lx < vx; ++lx) {
+        if (lx + 1 < vx) load_psi(a.psi_e, cE, ya_e, za_e, lx + 1, nxt);
+        const int c = lx * plane + col;
+        const int x = X0 + lx;
+        const int i = x * P + o;
+        const float hx = H[c], hy = H[cells + c], hz = H[2 * cells + c];
+        float yx, yz, zx, zy;
+        if (ly > 0) {
+          yx = H[c - sz];
+          yz = H[2 * cells + c - sz];
+        } else if (wyl >= 0) {
+          yx = FY[lx * sz + lz];
+          yz = FY[fxz + lx * sz + lz];
+        } else {
+          yx = yz = 0.f;
         }
-      };
-      pass(true);
-      __syncthreads();
-      send(myface + half, E, 0, 0, 0, 2u * k + 2u);  // low E faces for the low neighbours
-      pass(false);
+        if (lz > 0) {
+          zx = H[c - 1];
+          zy = H[cells + c - 1];
+        } else if (wzl >= 0) {
+          zx = FZ[lx * sy + ly];
+          zy = FZ[fxy + lx * sy + ly];
+        } else {
+          zx = zy = 0.f;
+        }
+        const float dx1 = hy - my, dx2 = hz - mz;
+        const float dy0 = hx - yx, dy2 = hz - yz;
+        const float dz0 = hx - zx, dz1 = hy - zy;
+        const float xa = cE[lx * 3], xb = cE[lx * 3 + 1], xk = cE[lx * 3 + 2];
+        float* psi = a.psi_e + i;
+        const float dxz = cpml_pre(dx2, xa, xb, xk, cur[0], psi);
+        const float dxy = cpml_pre(dx1, xa, xb, xk, cur[1], psi + n);
+        const float dyx = cpml_pre(dy0, ya_e, yb_e, yk_e, cur[2], psi + 2 * n);
+        const float dyz = cpml_pre(dy2, ya_e, yb_e, yk_e, cur[3], psi + 3 * n);
+        const float dzy = cpml_pre(dz1, za_e, zb_e, zk_e, cur[4], psi + 4 * n);
+        const float dzx = cpml_pre(dz0, za_e, zb_e, zk_e, cur[5], psi + 5 * n);
+        const float ie = IE[c];
+        float ex = E[c] + (S * (dyz - dzy)) * ie;
+        float ey = E[cells + c] + (S * (dzx - dxz)) * ie;
+        float ez = E[2 * cells + c] + (S * (dxy - dyx)) * ie;
+        if (src_here && x == a.source_x) {
+          ey += ((a.src_h[2 * P + o] * wave_y) * S) * ie;
+          ez -= ((a.src_h[P + o] * wave_z) * S) * ie;
+        }
+        E[c] = ex;
+        E[cells + c] = ey;
+        E[2 * cells + c] = ez;
+        my = hy;
+        mz = hz;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) cur[m] = nxt[m];
+      }
     }
     __syncthreads();
+    send(myface + half, E, 0, 0, 0, 2u * k + 2u);  // low E faces for the low neighbours
     // ------------------------------------------------------------ H half
     if (!receive(wxh, wyh, wzh, half, 2u * k + 2u)) return;
-    {
+    if (column) {
       const bool src_here = a.source_x >= X0 && a.source_x < X0 + vx && t < a.steps;
       const float wave_y = src_here ? a.wave_e[2 * a.steps + t] : 0.f;
       const float wave_z = src_here ? a.wave_e[a.steps + t] : 0.f;
       const int slot = a.nstage ? t % a.slots : 0;
-      auto pass = [&](bool boundary) {
+      float ex = E[col], ey = E[cells + col], ez = E[2 * cells + col];  // plane lx
+      float cur[6], nxt[6];
+      load_psi(a.psi_h, cH, ya_h, za_h, 0, cur);
 #pragma unroll 1
-        for (int c = tid; c < cells; c += kThreads) {
-          int lx, ly, lz;
-          coords(c, lx, ly, lz);
-          if (lx >= vx || ly >= vy || lz >= vz) continue;
-          if ((lx == vx - 1 || ly == vy - 1 || lz == vz - 1) != boundary) continue;
-          const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
-          const int i = (x * ny + y) * nz + z;
-          const float ex = E[c], ey = E[cells + c], ez = E[2 * cells + c];
-          float py, pz, qx, qz, rx, ry;
-          if (lx + 1 < vx) {
-            py = E[cells + c + sy * sz];
-            pz = E[2 * cells + c + sy * sz];
-          } else if (wxh >= 0) {
-            py = FX[ly * sz + lz];
-            pz = FX[fyz + ly * sz + lz];
-          } else {
-            py = pz = 0.f;
-          }
-          if (ly + 1 < vy) {
-            qx = E[c + sz];
-            qz = E[2 * cells + c + sz];
-          } else if (wyh >= 0) {
-            qx = FY[lx * sz + lz];
-            qz = FY[fxz + lx * sz + lz];
-          } else {
-            qx = qz = 0.f;
-          }
-          if (lz + 1 < vz) {
-            rx = E[c + 1];
-            ry = E[cells + c + 1];
-          } else if (wzh >= 0) {
-            rx = FZ[lx * sy + ly];
-            ry = FZ[fxy + lx * sy + ly];
-          } else {
-            rx = ry = 0.f;
-          }
-          const float dx1 = py - ey, dx2 = pz - ez;
-          const float dy0 = qx - ex, dy2 = qz - ez;
-          const float dz0 = rx - ex, dz1 = ry - ey;
-          const float* cx = cH + lx * 3;
-          const float* cy = cH + (sx + ly) * 3;
-          const float* cz = cH + (sx + sy + lz) * 3;
-          float* psi = a.psi_h + i;
-          const float dxz = cpml_step(dx2, cx[0], cx[1], cx[2], psi);
-          const float dxy = cpml_step(dx1, cx[0], cx[1], cx[2], psi + n);
-          const float dyx = cpml_step(dy0, cy[0], cy[1], cy[2], psi + 2 * n);
-          const float dyz = cpml_step(dy2, cy[0], cy[1], cy[2], psi + 3 * n);
-          const float dzy = cpml_step(dz1, cz[0], cz[1], cz[2], psi + 4 * n);
-          const float dzx = cpml_step(dz0, cz[0], cz[1], cz[2], psi + 5 * n);
-          const float ox = H[c], oy = H[cells + c], oz = H[2 * cells + c];
-          float hx = ox - S * (dyz - dzy);
-          float hy = oy - S * (dzx - dxz);
-          float hz = oz - S * (dxy - dyx);
-          if (src_here && x == a.source_x) {
-            const int o = y * nz + z;
-            hy -= (a.src_e[2 * P + o] * wave_y) * S;
-            hz += (a.src_e[P + o] * wave_z) * S;
-          }
-          H[c] = hx;
-          H[cells + c] = hy;
-          H[2 * cells + c] = hz;
-          for (int b = 0; b < a.nstage; ++b) {
-            const StageBox& s = a.stages[b];
-            if (x >= s.x0 && x < s.x1 && y >= s.y0 && y < s.y1 && z >= s.z0 && z < s.z1) {
-              const int cnt = s.count;
-              float* d = s.buf + (size_t)slot * 9 * cnt +
-                         ((x - s.x0) * (s.y1 - s.y0) + (y - s.y0)) * (s.z1 - s.z0) + (z - s.z0);
-              d[0] = ex;
-              d[cnt] = ey;
-              d[2 * cnt] = ez;
-              d[3 * cnt] = hx;
-              d[4 * cnt] = hy;
-              d[5 * cnt] = hz;
-              d[6 * cnt] = ox;
-              d[7 * cnt] = oy;
-              d[8 * cnt] = oz;
-            }
+      for (int lx = 0; lx < vx; ++lx) {
+        if (lx + 1 < vx) load_psi(a.psi_h, cH, ya_h, za_h, lx + 1, nxt);
+        const int c = lx * plane + col;
+        const int x = X0 + lx;
+        const int i = x * P + o;
+        float py, pz;  // E_y, E_z of plane lx + 1
+        float px = 0.f;
+        if (lx + 1 < vx) {
+          px = E[c + plane];
+          py = E[cells + c + plane];
+          pz = E[2 * cells + c + plane];
+        } else if (wxh >= 0) {
+          py = FX[ly * sz + lz];
+          pz = FX[fyz + ly * sz + lz];
+        } else {
+          py = pz = 0.f;
+        }
+        float qx, qz, rx, ry;
+        if (ly + 1 < vy) {
+          qx = E[c + sz];
+          qz = E[2 * cells + c + sz];
+        } else if (wyh >= 0) {
+          qx = FY[lx * sz + lz];
+          qz = FY[fxz + lx * sz + lz];
+        } else {
+          qx = qz = 0.f;
+        }
+        if (lz + 1 < vz) {
+          rx = E[c + 1];
+          ry = E[cells + c + 1];
+        } else if (wzh >= 0) {
+          rx = FZ[lx * sy + ly];
+          ry = FZ[fxy + lx * sy + ly];
+        } else {
+          rx = ry = 0.f;
+        }
+        const float dx1 = py - ey, dx2 = pz - ez;
+        const float dy0 = qx - ex, dy2 = qz - ez;
+        const float dz0 = rx - ex, dz1 = ry - ey;
+        const float xa = cH[lx * 3], xb = cH[lx * 3 + 1], xk = cH[lx * 3 + 2];
+        float* psi = a.psi_h + i;
+        const float dxz = cpml_pre(dx2, xa, xb, xk, cur[0], psi);
+        const float dxy = cpml_pre(dx1, xa, xb, xk, cur[1], psi + n);
+        const float dyx = cpml_pre(dy0, ya_h, yb_h, yk_h, cur[2], psi + 2 * n);
+        const float dyz = cpml_pre(dy2, ya_h, yb_h, yk_h, cur[3], psi + 3 * n);
+        const float dzy = cpml_pre(dz1, za_h, zb_h, zk_h, cur[4], psi + 4 * n);
+        const float dzx = cpml_pre(dz0, za_h, zb_h, zk_h, cur[5], psi + 5 * n);
+        const float ox = H[c], oy = H[cells + c], oz = H[2 * cells + c];
+        float hx = ox - S * (dyz - dzy);
+        float hy = oy - S * (dzx - dxz);
+        float hz = oz - S * (dxy - dyx);
+        if (src_here && x == a.source_x) {
+          hy -= (a.src_e[2 * P + o] * wave_y) * S;
+          hz += (a.src_e[P + o] * wave_z) * S;
+        }
+        H[c] = hx;
+        H[cells + c] = hy;
+        H[2 * cells + c] = hz;
+        for (int b = 0; b < a.nstage; ++b) {
+          const StageBox& st = a.stages[b];
+          if (x >= st.x0 && x < st.x1 && y >= st.y0 && y < st.y1 && z >= st.z0 && z < st.z1) {
+            const int cnt = st.count;
+            float* d = st.buf + (size_t)slot * 9 * cnt +
+                       ((x - st.x0) * (st.y1 - st.y0) + (y - st.y0)) * (st.z1 - st.z0) +
+                       (z - st.z0);
+            d[0] = ex;
+            d[cnt] = ey;
+            d[2 * cnt] = ez;
+            d[3 * cnt] = hx;
+            d[4 * cnt] = hy;
+            d[5 * cnt] = hz;
+            d[6 * cnt] = ox;
+            d[7 * cnt] = oy;
+            d[8 * cnt] = oz;
           }
         }
-      };
-      pass(true);
-      __syncthreads();
-      send(myface, H, vx - 1, vy - 1, vz - 1, 2u * k + 3u);  // high H faces
-      pass(false);
+        ex = px;
+        ey = py;
+        ez = pz;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) cur[m] = nxt[m];
+      }
     }
     __syncthreads();
+    send(myface, H, vx - 1, vy - 1, vz - 1, 2u * k + 3u);  // high H faces
   }
 #pragma unroll 1
   for (int c = tid; c < cells; c += kThreads) {
@@ -504,9 +538,9 @@ struct Layout {
   int sx, sy, sz, gx, gy, gz, blocks;
 };
 
-// Sub-box choice: the fewest cell passes per workgroup (ceil(cells / 1024)),
-// then the smaller sub-box surface (face traffic), then fewer workgroups;
-// every candidate fits the LDS (lds_floats <= kLds).
+// Sub-box choice: one thread per (y, z) column (sy*sz <= 1024), the fewest x
+// planes per column, then the smaller sub-box surface (face traffic), then
+// fewer workgroups; every candidate fits the LDS (lds_floats <= kLds).
 bool choose(int nx, int ny, int nz, int capacity, int want_sx, int want_sy, int want_sz,
             Layout& best) {
   bool found = false;
@@ -515,21 +549,20 @@ bool choose(int nx, int ny, int nz, int capacity, int want_sx, int want_sy, int 
     if (want_sz && sz != want_sz) continue;
     for (int sy = 1; sy <= ny; ++sy) {
       if (want_sy && sy != want_sy) continue;
-      if (7L * sy * sz > kLds) break;
+      if (7L * sy * sz > kLds || sy * sz > kThreads) break;
       const int top = min(nx, (int)(kLds / (7L * sy * sz)));
       for (int sx = want_sx ? want_sx : 1; sx <= (want_sx ? want_sx : top); ++sx) {
         // even out the cells per workgroup along each axis without adding
         // workgroups: an evened size never exceeds the first one
         int gx = (nx + sx - 1) / sx, gy = (ny + sy - 1) / sy, gz = (nz + sz - 1) / sz;
         const int ex = (nx + gx - 1) / gx, ey = (ny + gy - 1) / gy, ez = (nz + gz - 1) / gz;
-        if (lds_floats(ex, ey, ez) > kLds) continue;
+        if (lds_floats(ex, ey, ez) > kLds || ey * ez > kThreads) continue;
         gx = (nx + ex - 1) / ex;
         gy = (ny + ey - 1) / ey;
         gz = (nz + ez - 1) / ez;
         const long blocks = (long)gx * gy * gz;
         if (blocks > capacity) continue;
-        const long key[3] = {((long)ex * ey * ez + kThreads - 1) / kThreads,
-                             (long)ex * ey + (long)ex * ez + (long)ey * ez, blocks};
+        const long key[3] = {ex, (long)ex * ey + (long)ex * ez + (long)ey * ez, blocks};
         const bool better = !found || key[0] < best_key[0] ||
                             (key[0] == best_key[0] &&
                              (key[1] < best_key[1] ||
@@ -614,6 +647,7 @@ int wfd_resident_run(const wfd_yee_plan* p, const wfd_resident_layout* L, void* 
               WFD_MAX_STAGES);
   WFD_REQUIRE(p->nstage == 0 || p->slots > 0, "resident_run: stage slots must be positive");
   WFD_REQUIRE(L->sx > 0 && L->sy > 0 && L->sz > 0 && lds_floats(L->sx, L->sy, L->sz) <= kLds &&
+                  L->sy * L->sz <= kThreads &&
                   L->gx == (p->nx + L->sx - 1) / L->sx &&
                   L->gy == (p->ny + L->sy - 1) / L->sy &&
                   L->gz == (p->nz + L->sz - 1) / L->sz && L->blocks == L->gx * L->gy * L->gz &&
