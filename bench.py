@@ -136,6 +136,11 @@ def copy_probe(mib):
     print(json.dumps({"copy_mib": mib, "copies": 8}), flush=True)
 
 
+def log(message):
+    """Progress line on stderr (long workloads)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {message}", file=sys.stderr, flush=True)
+
+
 def _line(metric, value, unit, steps, warmup, ms, config, dtype="f32", **extra):
     line = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
             "warmup": warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -202,11 +207,15 @@ def bench_batch(args):
     pkg = __graft_entry__.package()
     root = Path(tempfile.gettempdir()) / "wfdtd_bench_batch32"
     t0 = time.perf_counter()
+    log("writing 32 design bundles")
     cases = pkg.cases.batch_cases(root, count=32, seed=0, steps=3000)
     gen = time.perf_counter() - t0
+    log(f"bundles ready ({gen:.0f} s); compiling")
     bt = pkg.design.BatchTransmission([c for c, _ in cases], wavelength=1.55e-6,
                                       checkpoint_every=192)
+    bt.log = log
     bt.compile()
+    log("batched forward")
     rhos = [r for _, r in cases]
     t0 = time.perf_counter()
     T = bt.forward(rhos)
@@ -237,8 +246,9 @@ def bench_surrogate(args):
     sg = pkg.surrogate
     path = Path(tempfile.gettempdir()) / "wfdtd_bench_surrogate100"
     if not (path / "manifest.json").exists():
+        log("writing the surrogate bend bundle")
         sg.surrogate_case(path, side=100)
-    ev = sg.Evaluator(path, batch=32, checkpoint_every=192)
+    ev = sg.Evaluator(path, batch=32, checkpoint_every=192, log=log)
     t0 = time.perf_counter()
     rhos = sg.random_designs(args.designs, 100, seed=0)
     T, G = ev.label(rhos)
@@ -247,14 +257,14 @@ def bench_surrogate(args):
     Th, Gh = ev.label(held)
     net = pkg.fno.FNO(100, batch=32, seed=0)
     t0 = time.perf_counter()
-    norm, hist = sg.train(net, rhos, T, G, epochs=args.epochs, lr=2e-3)
+    norm, hist = sg.train(net, rhos, T, G, epochs=args.epochs, lr=2e-3, log=log)
     train_s = time.perf_counter() - t0
     f = sg.objective_index()
     pred = np.array([sg.predict(net, norm, r)[0] for r in held])
     rel = np.abs(pred[:, f] - Th[:, f]) / np.abs(Th[:, f])
     rho0 = sg.random_designs(1, 100, seed=7)[0]
-    plain = sg.optimize(ev, rho0, steps=args.opt_steps, step=0.05)
-    sur = sg.optimize(ev, rho0, steps=args.opt_steps, step=0.05, surrogate=(net, norm))
+    plain = sg.optimize(ev, rho0, steps=args.opt_steps, step=0.05, log=log)
+    sur = sg.optimize(ev, rho0, steps=args.opt_steps, step=0.05, surrogate=(net, norm), log=log)
     torch.cuda.synchronize()
     within = float(np.mean(rel <= 0.005))
     per_hour = args.opt_steps / sur["seconds"] * 3600

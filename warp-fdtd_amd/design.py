@@ -190,11 +190,14 @@ class BatchTransmission:
         self.steps = self.batch.steps
         self._compiled = False
         self.timings = {}
+        self.log = lambda message: None
 
     def compile(self):
         if not self._compiled:
-            for p in self.problems:
+            for k, p in enumerate(self.problems):
                 p._compile()
+                if k % 8 == 7:
+                    self.log(f"compiled {k + 1}/{len(self.problems)} designs")
             self.batch.compile()
             self._compiled = True
 
@@ -231,6 +234,10 @@ class BatchTransmission:
     def backward(self):
         """dT/drho of every design for the densities of the last forward."""
         t0 = time.perf_counter()
-        grads = [p.backward(p._forward_rho) for p in self.problems]
+        grads = []
+        for k, p in enumerate(self.problems):
+            grads.append(p.backward(p._forward_rho))
+            if k % 8 == 7:
+                self.log(f"adjoint {k + 1}/{len(self.problems)}")
         self.timings["backward_s"] = time.perf_counter() - t0
         return grads

@@ -45,12 +45,15 @@ def objective_index():
 class Evaluator:
     """FDTD evaluations of densities on one surrogate bundle."""
 
-    def __init__(self, case, batch=32, checkpoint_every=192, device="cuda:0"):
+    def __init__(self, case, batch=32, checkpoint_every=192, device="cuda:0", log=None):
+        self.log = log or (lambda message: None)
         self.case = case
         self.wavelength = float(WAVELENGTHS[objective_index()])
         self.batch = batch
         self.labeler = BatchTransmission([case] * batch, wavelength=self.wavelength,
                                          checkpoint_every=checkpoint_every, device=device)
+        self.labeler.log = self.log
+        self.log(f"compiling {batch} batched adjoints")
         self.labeler.compile()
         self.single = self.labeler.problems[0]
         self.checker = Solver(case, device=device)
@@ -69,6 +72,7 @@ class Evaluator:
             pad = chunk + [chunk[-1]] * (self.batch - len(chunk))
             self.labeler.forward(pad)
             g = self.labeler.backward()
+            self.log(f"labelled {min(start + self.batch, len(rhos))}/{len(rhos)} designs")
             for k in range(len(chunk)):
                 spectra.append(self.labeler.problems[k].adjoint.transmission_spectrum())
                 grads.append(g[k].cpu().numpy())
@@ -167,7 +171,7 @@ def _ascent(rho, g, step):
     return np.clip(rho + step * g / scale, 0.0, 1.0)
 
 
-def optimize(evaluator, rho0, steps=200, step=0.05, surrogate=None, tolerance=0.01):
+def optimize(evaluator, rho0, steps=200, step=0.05, surrogate=None, tolerance=0.01, log=None):
     """Gradient ascent on T at the objective wavelength.  ``surrogate`` =
     (net, norm) proposes each step from the FNO's gradient; every proposal is
     re-verified by a full FDTD forward, and a proposal whose surrogate T is off
@@ -178,10 +182,12 @@ def optimize(evaluator, rho0, steps=200, step=0.05, surrogate=None, tolerance=0.
     start = time.perf_counter()
     history, fallbacks = [], 0
     if surrogate is None:
-        for _ in range(steps):
+        for k in range(steps):
             spec, g = evaluator.adjoint(rho)
             history.append(float(spec[f]))
             rho = _ascent(rho, g, step)
+            if log and k % 20 == 0:
+                log(f"adjoint step {k}: T = {history[-1]:.5f}")
         final = float(evaluator.transmission(rho)[f])
     else:
         net, norm = surrogate
@@ -198,6 +204,8 @@ def optimize(evaluator, rho0, steps=200, step=0.05, surrogate=None, tolerance=0.
                 t_true = float(evaluator.transmission(proposal)[f])
             rho, current = proposal, t_true
             history.append(current)
+            if log and len(history) % 20 == 1:
+                log(f"surrogate step {len(history) - 1}: T = {current:.5f}, fallbacks {fallbacks}")
         final = current
     torch.cuda.synchronize()
     return {"final": final, "history": history, "rho": rho, "fallbacks": fallbacks,
