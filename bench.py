@@ -36,6 +36,12 @@ STEPS = 4000
 HBM_PEAK_GBS = 8000.0
 
 
+# HBM bytes per step measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
+# workload (scaled by a 256 MiB copy: FETCH x2.000, WRITE x1.000), see
+# profiles/hbm_traffic_waveguide_160x96x80.md; None for other shapes.
+MEASURED_TRAFFIC = {(SHAPE, True): 66.14e6, (SHAPE, False): 130.9e6}
+
+
 def algorithmic_bytes_per_cell(shape, pml):
     f_cpml = sum(2 * pml / n for n in shape)
     return 52.0 + 32.0 * f_cpml, f_cpml
@@ -349,8 +355,13 @@ def main():
     B, f_cpml = algorithmic_bytes_per_cell(s.shape, PML)
     median_s, mean_s = kernel_time_per_step(pkg, s, args.kernel_repeats)
     achieved = cells * B / mean_s / 1e9
+    traffic = MEASURED_TRAFFIC.get((tuple(s.shape), bool(s.resident)))
+    launch_steps = 64 if s.resident else 1
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None if traffic is None else round(traffic * launch_steps),
+                "traffic_per_step_bytes": traffic,
+                "algorithmic_bytes_per_step": round(cells * B),
                 "kernel": ("k_resident (steps per launch: 64)" if s.resident
                            else "k_yee_e + k_yee_h (one fused step)"),
                 "bytes_per_cell_update": round(B, 2), "f_cpml": round(f_cpml, 4),
