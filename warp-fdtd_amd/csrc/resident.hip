@@ -62,11 +62,10 @@ struct ResArgs {
   int spin_limit;
 };
 
-// LDS floats a sub-box needs: E, H, 1/eps per cell, (a, b, 1/kappa) rows of E
-// and H along each axis, and two components of the three incoming faces.
+// LDS floats a sub-box needs: E, H, 1/eps per cell and (a, b, 1/kappa) rows
+// of E and H along each axis.
 __host__ __device__ inline long lds_floats(int sx, int sy, int sz) {
-  return 7L * sx * sy * sz + 6L * (sx + sy + sz) +
-         2L * ((long)sy * sz + (long)sx * sz + (long)sx * sy);
+  return 7L * sx * sy * sz + 6L * (sx + sy + sz);
 }
 
 // Words of one workgroup's face buffer: H high faces then E low faces, two
@@ -134,9 +133,6 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
   float* H = lds + 3 * cells;     // [3][cells]
   float* IE = lds + 6 * cells;    // [cells]
   float* co = lds + 7 * cells;    // [2][R][3]: E rows then H rows (x | y | z)
-  float* FX = co + 6 * R;         // incoming x face [2][sy*sz]
-  float* FY = FX + 2 * fyz;       // incoming y face [2][sx*sz]
-  float* FZ = FY + 2 * fxz;       // incoming z face [2][sx*sy]
   const int ny = a.ny, nz = a.nz, P = ny * nz, n = a.n;
   const int sxy = a.gz * a.gy;
   const int wxl = bxi > 0 ? wg - sxy : -1, wxh = bxi + 1 < a.gx ? wg + sxy : -1;
@@ -203,133 +199,60 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
   }
   __syncthreads();
 
-  // Outgoing faces of field F at planes (px, py, pz), components (1,2) on x,
-  // (0,2) on y, (0,1) on z, into words at f with tag.
-  auto send = [&](Word* f, const float* F, int px, int py, int pz, unsigned tag) {
-    for (int t = tid; t < 2 * vy * vz; t += kThreads) {
-      const int comp = t >= vy * vz, r = t - comp * vy * vz;
-      const int ly = r / vz, lz = r - ly * vz;
-      put(f + comp * fyz + ly * sz + lz, F[(1 + comp) * cells + (px * sy + ly) * sz + lz], tag);
-    }
-    for (int t = tid; t < 2 * vx * vz; t += kThreads) {
-      const int comp = t >= vx * vz, r = t - comp * vx * vz;
-      const int lx = r / vz, lz = r - lx * vz;
-      put(f + 2 * fyz + comp * fxz + lx * sz + lz,
-          F[2 * comp * cells + (lx * sy + py) * sz + lz], tag);
-    }
-    for (int t = tid; t < 2 * vx * vy; t += kThreads) {
-      const int comp = t >= vx * vy, r = t - comp * vx * vy;
-      const int lx = r / vy, ly = r - lx * vy;
-      put(f + 2 * fyz + 2 * fxz + comp * fxy + lx * sy + ly,
-          F[comp * cells + (lx * sy + ly) * sz + pz], tag);
-    }
-  };
-  // Incoming faces (words at neighbour buffers + off) into FX, FY, FZ;
-  // false, uniformly, when a poll gave up.  The words of the three faces form
-  // one index space; each thread issues up to kBatch loads before it checks
-  // any, so the polls of a half-step cost one round trip, not three.
-  auto receive = [&](int nxw, int nyw, int nzw, size_t off, unsigned tag) {
-    const int cx_ = nxw >= 0 ? 2 * vy * vz : 0;
-    const int cy_ = nyw >= 0 ? 2 * vx * vz : 0;
-    const int cz_ = nzw >= 0 ? 2 * vx * vy : 0;
-    const int total = cx_ + cy_ + cz_;
-    auto locate = [&](int u, const Word*& src, float*& dst) {
-      if (u < cx_) {
-        const int comp = u >= vy * vz, r = u - comp * vy * vz;
-        const int ly = r / vz, lz = r - ly * vz;
-        const int k = comp * fyz + ly * sz + lz;
-        src = a.face + (size_t)nxw * per + off + k;
-        dst = FX + k;
-      } else if (u < cx_ + cy_) {
-        u -= cx_;
-        const int comp = u >= vx * vz, r = u - comp * vx * vz;
-        const int lx = r / vz, lz = r - lx * vz;
-        const int k = comp * fxz + lx * sz + lz;
-        src = a.face + (size_t)nyw * per + off + 2 * fyz + k;
-        dst = FY + k;
-      } else {
-        u -= cx_ + cy_;
-        const int comp = u >= vx * vy, r = u - comp * vx * vy;
-        const int lx = r / vy, ly = r - lx * vy;
-        const int k = comp * fxy + lx * sy + ly;
-        src = a.face + (size_t)nzw * per + off + 2 * fyz + 2 * fxz + k;
-        dst = FZ + k;
-      }
-    };
-    bool ok = true;
-    // one lane per wavefront waits for the wave's first word (the other 63
-    // lanes are masked off meanwhile), so a waiting workgroup keeps 16 polls
-    // in flight instead of 1024
-    {
-      const int u0 = tid & ~63;
-      if ((tid & 63) == 0 && u0 < total) {
-        const Word* src0;
-        float* dst0;
-        locate(u0, src0, dst0);
-        ok = take(a, src0, tag, *dst0);
-      }
-    }
-    for (int base = 0; ok && base < total; base += kBatch * kThreads) {
-      const Word* src[kBatch];
-      float* dst[kBatch];
-      Word w[kBatch];
-#pragma unroll
-      for (int q = 0; q < kBatch; ++q) {
-        const int u = base + q * kThreads + tid;
-        src[q] = nullptr;
-        if (u < total) {
-          locate(u, src[q], dst[q]);
-          w[q] = __hip_atomic_load(src[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < kBatch; ++q) {
-        if (!src[q] || !ok) continue;
-        if ((unsigned)(w[q] >> 32) == tag) {
-          *dst[q] = __uint_as_float((unsigned)w[q]);
-        } else {
-          ok = take(a, src[q], tag, *dst[q]);
-        }
-      }
-    }
-    if (!ok) bail = 1;
-    __syncthreads();
-    return bail == 0;
-  };
-
-  send(myface, H, vx - 1, vy - 1, vz - 1, 1u);
-
-  // Compute: thread t < sy*sz owns the column (ly, lz) = (t / sz, t % sz) of
-  // the sub-box and marches lx upward, so the x neighbour comes from the
-  // previous (E half) or next (H half) plane in registers, the y and z CPML
-  // rows are per-thread constants, and the psi values of the next plane are
-  // loaded while the current one is updated.
-  const int ly = tid / sz, lz = tid - (tid / sz) * sz;
-  const bool column = tid < sy * sz && ly < vy && lz < vz;
-  const int y = Y0 + ly, z = Z0 + lz;
-  const int col = ly * sz + lz;  // cell index of the column at lx = 0
-  const int plane = sy * sz;      // cell stride of lx
-  const int o = y * nz + z;       // source-plane offset
-  float ya_e = 0.f, yb_e = 0.f, yk_e = 0.f, za_e = 0.f, zb_e = 0.f, zk_e = 0.f;
-  float ya_h = 0.f, yb_h = 0.f, yk_h = 0.f, za_h = 0.f, zb_h = 0.f, zk_h = 0.f;
-  if (column) {
-    ya_e = cE[(sx + ly) * 3];
-    yb_e = cE[(sx + ly) * 3 + 1];
-    yk_e = cE[(sx + ly) * 3 + 2];
-    za_e = cE[(sx + sy + lz) * 3];
-    zb_e = cE[(sx + sy + lz) * 3 + 1];
-    zk_e = cE[(sx + sy + lz) * 3 + 2];
-    ya_h = cH[(sx + ly) * 3];
-    yb_h = cH[(sx + ly) * 3 + 1];
-    yk_h = cH[(sx + ly) * 3 + 2];
-    za_h = cH[(sx + sy + lz) * 3];
-    zb_h = cH[(sx + sy + lz) * 3 + 1];
-    zk_h = cH[(sx + sy + lz) * 3 + 2];
+  // The initial H faces (tag 1): components (1,2) on x, (0,2) on y, (0,1) on z.
+  for (int t = tid; t < 2 * vy * vz; t += kThreads) {
+    const int comp = t >= vy * vz, r = t - comp * vy * vz;
+    const int ly = r / vz, lz = r - ly * vz;
+    put(myface + comp * fyz + ly * sz + lz,
+        H[(1 + comp) * cells + ((vx - 1) * sy + ly) * sz + lz], 1u);
   }
-  // psi of plane lx (six slots, zero where the axis has no CPML there)
-  auto load_psi = [&](const float* psi, const float* cf, float ya, float za, int lx,
+  for (int t = tid; t < 2 * vx * vz; t += kThreads) {
+    const int comp = t >= vx * vz, r = t - comp * vx * vz;
+    const int lx = r / vz, lz = r - lx * vz;
+    put(myface + 2 * fyz + comp * fxz + lx * sz + lz,
+        H[2 * comp * cells + (lx * sy + vy - 1) * sz + lz], 1u);
+  }
+  for (int t = tid; t < 2 * vx * vy; t += kThreads) {
+    const int comp = t >= vx * vy, r = t - comp * vx * vy;
+    const int lx = r / vy, ly = r - lx * vy;
+    put(myface + 2 * fyz + 2 * fxz + comp * fxy + lx * sy + ly,
+        H[comp * cells + (lx * sy + ly) * sz + vz - 1], 1u);
+  }
+
+  // Columns: thread t < vy*vz owns one (y, z) column per half-step and marches
+  // x.  The columns that read neighbour faces in y or z are the first
+  // vy + vz - 1 threads (one wavefront for sub-boxes with vy + vz <= 65), so the
+  // other wavefronts never wait on them; every face word is polled by the
+  // thread that uses it, right when it needs it, and sent by the thread that
+  // computed it, right after computing it.  Per-word dependencies order every
+  // overwrite after the read of the previous value (see the file header), so
+  // the half-steps need no barrier beyond the LDS one between them.
+  const int ncol = vy * vz;
+  const bool active = tid < ncol;
+  const int nb = vy + vz - 1, w1 = vz - 1;
+  int eyl, ezl, hyl, hzl;
+  if (tid < vz) {
+    eyl = 0;
+    ezl = tid;
+    hyl = vy - 1;
+    hzl = tid;
+  } else if (tid < nb) {
+    eyl = tid - vz + 1;
+    ezl = 0;
+    hyl = tid - vz;
+    hzl = vz - 1;
+  } else {
+    const int m = tid - nb;
+    eyl = w1 > 0 ? 1 + m / w1 : 0;
+    ezl = w1 > 0 ? 1 + m % w1 : 0;
+    hyl = w1 > 0 ? m / w1 : 0;
+    hzl = w1 > 0 ? m % w1 : 0;
+  }
+  const int plane = sy * sz;
+
+  // psi of one plane of a column (zero where the axis has no CPML there)
+  auto load_psi = [&](const float* psi, const float* cf, float ya, float za, int i, int lx,
                       float (&p)[6]) {
-    const int i = (X0 + lx) * P + o;
     const bool ax = cf[lx * 3] != 0.f;
     p[0] = ax ? psi[i] : 0.f;
     p[1] = ax ? psi[n + i] : 0.f;
@@ -338,61 +261,112 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
     p[4] = za != 0.f ? psi[4 * n + i] : 0.f;
     p[5] = za != 0.f ? psi[5 * n + i] : 0.f;
   };
+  // a face value: the word already loaded if it carries the tag, else poll
+  auto value = [&](const Word* p, Word w, unsigned tag) {
+    if ((unsigned)(w >> 32) == tag) return __uint_as_float((unsigned)w);
+    float v = 0.f;
+    if (!take(a, p, tag, v)) bail = 1;
+    return v;
+  };
+  auto peek = [&](const Word* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  };
 
   for (int k = 0; k < nsteps; ++k) {
     const int t = t0 + k;
     // ------------------------------------------------------------ E half
-    if (!receive(wxl, wyl, wzl, 0, 2u * k + 1u)) return;
-    if (column) {
+    if (active) {
+      const unsigned tag = 2u * k + 1u, out = 2u * k + 2u;
+      const int ly = eyl, lz = ezl, col = ly * sz + lz;
+      const int y = Y0 + ly, z = Z0 + lz, o = y * nz + z;
+      const float ya = cE[(sx + ly) * 3], yb = cE[(sx + ly) * 3 + 1], yk = cE[(sx + ly) * 3 + 2];
+      const float za = cE[(sx + sy + lz) * 3], zb = cE[(sx + sy + lz) * 3 + 1],
+                  zk = cE[(sx + sy + lz) * 3 + 2];
+      const bool fy = ly == 0 && wyl >= 0, fz = lz == 0 && wzl >= 0;
+      const Word* px = wxl >= 0 ? a.face + (size_t)wxl * per + ly * sz + lz : nullptr;
+      const Word* py = fy ? a.face + (size_t)wyl * per + 2 * fyz + lz : nullptr;
+      const Word* pz = fz ? a.face + (size_t)wzl * per + 2 * fyz + 2 * fxz + ly : nullptr;
       const bool src_here = a.source_x >= X0 && a.source_x < X0 + vx && t < a.steps;
       const float wave_y = src_here ? a.wave_h[2 * a.steps + t] : 0.f;
       const float wave_z = src_here ? a.wave_h[a.steps + t] : 0.f;
-      float my, mz;  // H_y, H_z of plane lx - 1
-      if (wxl >= 0) {
-        my = FX[ly * sz + lz];
-        mz = FX[fyz + ly * sz + lz];
-      } else {
-        my = mz = 0.f;
-      }
+      // march lx = 1, 2, ..., vx - 1, then 0 (the plane that needs the x face)
+      float my = vx > 1 ? H[cells + col] : 0.f, mz = vx > 1 ? H[2 * cells + col] : 0.f;
+      int lx = vx > 1 ? 1 : 0;
       float cur[6], nxt[6];
-      load_psi(a.psi_e, cE, ya_e, za_e, 0, cur);
+      Word wy[2] = {0, 0}, wz[2] = {0, 0};
+      load_psi(a.psi_e, cE, ya, za, (X0 + lx) * P + o, lx, cur);
+      if (fy) {
+        wy[0] = peek(py + lx * sz);
+        wy[1] = peek(py + fxz + lx * sz);
+      }
+      if (fz) {
+        wz[0] = peek(pz + lx * sy);
+        wz[1] = peek(pz + fxy + lx * sy);
+      }
+      Word wx0 = 0, wx1 = 0;
+      if (px) {
+        wx0 = peek(px);
+        wx1 = peek(px + fyz);
+      }
 #pragma unroll 1
-      for (int lx = 0; lx < vx; ++lx) {
-        if (lx + 1 < vx) load_psi(a.psi_e, cE, ya_e, za_e, lx + 1, nxt);
+      for (int q = 0; q < vx; ++q) {
+        const int nlx = q + 2 < vx ? lx + 1 : 0;  // next plane of the march
+        Word ny0 = 0, ny1 = 0, nz0 = 0, nz1 = 0;
+        if (q + 1 < vx) {
+          load_psi(a.psi_e, cE, ya, za, (X0 + nlx) * P + o, nlx, nxt);
+          if (fy) {
+            ny0 = peek(py + nlx * sz);
+            ny1 = peek(py + fxz + nlx * sz);
+          }
+          if (fz) {
+            nz0 = peek(pz + nlx * sy);
+            nz1 = peek(pz + fxy + nlx * sy);
+          }
+        }
         const int c = lx * plane + col;
         const int x = X0 + lx;
         const int i = x * P + o;
         const float hx = H[c], hy = H[cells + c], hz = H[2 * cells + c];
+        float xm_y, xm_z;
+        if (lx > 0) {
+          xm_y = my;
+          xm_z = mz;
+        } else if (px) {
+          xm_y = value(px, wx0, tag);
+          xm_z = value(px + fyz, wx1, tag);
+        } else {
+          xm_y = xm_z = 0.f;
+        }
         float yx, yz, zx, zy;
         if (ly > 0) {
           yx = H[c - sz];
           yz = H[2 * cells + c - sz];
-        } else if (wyl >= 0) {
-          yx = FY[lx * sz + lz];
-          yz = FY[fxz + lx * sz + lz];
+        } else if (fy) {
+          yx = value(py + lx * sz, wy[0], tag);
+          yz = value(py + fxz + lx * sz, wy[1], tag);
         } else {
           yx = yz = 0.f;
         }
         if (lz > 0) {
           zx = H[c - 1];
           zy = H[cells + c - 1];
-        } else if (wzl >= 0) {
-          zx = FZ[lx * sy + ly];
-          zy = FZ[fxy + lx * sy + ly];
+        } else if (fz) {
+          zx = value(pz + lx * sy, wz[0], tag);
+          zy = value(pz + fxy + lx * sy, wz[1], tag);
         } else {
           zx = zy = 0.f;
         }
-        const float dx1 = hy - my, dx2 = hz - mz;
+        const float dx1 = hy - xm_y, dx2 = hz - xm_z;
         const float dy0 = hx - yx, dy2 = hz - yz;
         const float dz0 = hx - zx, dz1 = hy - zy;
         const float xa = cE[lx * 3], xb = cE[lx * 3 + 1], xk = cE[lx * 3 + 2];
         float* psi = a.psi_e + i;
         const float dxz = cpml_pre(dx2, xa, xb, xk, cur[0], psi);
         const float dxy = cpml_pre(dx1, xa, xb, xk, cur[1], psi + n);
-        const float dyx = cpml_pre(dy0, ya_e, yb_e, yk_e, cur[2], psi + 2 * n);
-        const float dyz = cpml_pre(dy2, ya_e, yb_e, yk_e, cur[3], psi + 3 * n);
-        const float dzy = cpml_pre(dz1, za_e, zb_e, zk_e, cur[4], psi + 4 * n);
-        const float dzx = cpml_pre(dz0, za_e, zb_e, zk_e, cur[5], psi + 5 * n);
+        const float dyx = cpml_pre(dy0, ya, yb, yk, cur[2], psi + 2 * n);
+        const float dyz = cpml_pre(dy2, ya, yb, yk, cur[3], psi + 3 * n);
+        const float dzy = cpml_pre(dz1, za, zb, zk, cur[4], psi + 4 * n);
+        const float dzx = cpml_pre(dz0, za, zb, zk, cur[5], psi + 5 * n);
         const float ie = IE[c];
         float ex = E[c] + (S * (dyz - dzy)) * ie;
         float ey = E[cells + c] + (S * (dzx - dxz)) * ie;
@@ -404,72 +378,125 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
         E[c] = ex;
         E[cells + c] = ey;
         E[2 * cells + c] = ez;
+        // low E faces for the low neighbours' H half
+        Word* f = myface + half;
+        if (lx == 0) {
+          put(f + ly * sz + lz, ey, out);
+          put(f + fyz + ly * sz + lz, ez, out);
+        }
+        if (ly == 0) {
+          put(f + 2 * fyz + lx * sz + lz, ex, out);
+          put(f + 2 * fyz + fxz + lx * sz + lz, ez, out);
+        }
+        if (lz == 0) {
+          put(f + 2 * fyz + 2 * fxz + lx * sy + ly, ex, out);
+          put(f + 2 * fyz + 2 * fxz + fxy + lx * sy + ly, ey, out);
+        }
         my = hy;
         mz = hz;
+        lx = nlx;
+        wy[0] = ny0;
+        wy[1] = ny1;
+        wz[0] = nz0;
+        wz[1] = nz1;
 #pragma unroll
         for (int m = 0; m < 6; ++m) cur[m] = nxt[m];
       }
     }
     __syncthreads();
-    send(myface + half, E, 0, 0, 0, 2u * k + 2u);  // low E faces for the low neighbours
+    if (bail) return;
     // ------------------------------------------------------------ H half
-    if (!receive(wxh, wyh, wzh, half, 2u * k + 2u)) return;
-    if (column) {
+    if (active) {
+      const unsigned tag = 2u * k + 2u, out = 2u * k + 3u;
+      const int ly = hyl, lz = hzl, col = ly * sz + lz;
+      const int y = Y0 + ly, z = Z0 + lz, o = y * nz + z;
+      const float ya = cH[(sx + ly) * 3], yb = cH[(sx + ly) * 3 + 1], yk = cH[(sx + ly) * 3 + 2];
+      const float za = cH[(sx + sy + lz) * 3], zb = cH[(sx + sy + lz) * 3 + 1],
+                  zk = cH[(sx + sy + lz) * 3 + 2];
+      const bool fy = ly == vy - 1 && wyh >= 0, fz = lz == vz - 1 && wzh >= 0;
+      const Word* px = wxh >= 0 ? a.face + (size_t)wxh * per + half + ly * sz + lz : nullptr;
+      const Word* py = fy ? a.face + (size_t)wyh * per + half + 2 * fyz + lz : nullptr;
+      const Word* pz = fz ? a.face + (size_t)wzh * per + half + 2 * fyz + 2 * fxz + ly : nullptr;
       const bool src_here = a.source_x >= X0 && a.source_x < X0 + vx && t < a.steps;
       const float wave_y = src_here ? a.wave_e[2 * a.steps + t] : 0.f;
       const float wave_z = src_here ? a.wave_e[a.steps + t] : 0.f;
       const int slot = a.nstage ? t % a.slots : 0;
-      float ex = E[col], ey = E[cells + col], ez = E[2 * cells + col];  // plane lx
+      // march lx = 0, 1, ..., vx - 1 (the last plane needs the x face)
+      float ex = E[col], ey = E[cells + col], ez = E[2 * cells + col];
       float cur[6], nxt[6];
-      load_psi(a.psi_h, cH, ya_h, za_h, 0, cur);
+      Word wy[2] = {0, 0}, wz[2] = {0, 0};
+      load_psi(a.psi_h, cH, ya, za, X0 * P + o, 0, cur);
+      if (fy) {
+        wy[0] = peek(py);
+        wy[1] = peek(py + fxz);
+      }
+      if (fz) {
+        wz[0] = peek(pz);
+        wz[1] = peek(pz + fxy);
+      }
+      Word wx0 = 0, wx1 = 0;
+      if (px) {
+        wx0 = peek(px);
+        wx1 = peek(px + fyz);
+      }
 #pragma unroll 1
       for (int lx = 0; lx < vx; ++lx) {
-        if (lx + 1 < vx) load_psi(a.psi_h, cH, ya_h, za_h, lx + 1, nxt);
+        Word ny0 = 0, ny1 = 0, nz0 = 0, nz1 = 0;
+        if (lx + 1 < vx) {
+          load_psi(a.psi_h, cH, ya, za, (X0 + lx + 1) * P + o, lx + 1, nxt);
+          if (fy) {
+            ny0 = peek(py + (lx + 1) * sz);
+            ny1 = peek(py + fxz + (lx + 1) * sz);
+          }
+          if (fz) {
+            nz0 = peek(pz + (lx + 1) * sy);
+            nz1 = peek(pz + fxy + (lx + 1) * sy);
+          }
+        }
         const int c = lx * plane + col;
         const int x = X0 + lx;
         const int i = x * P + o;
-        float py, pz;  // E_y, E_z of plane lx + 1
-        float px = 0.f;
+        float xp_x = 0.f, xp_y, xp_z;  // E of plane lx + 1
         if (lx + 1 < vx) {
-          px = E[c + plane];
-          py = E[cells + c + plane];
-          pz = E[2 * cells + c + plane];
-        } else if (wxh >= 0) {
-          py = FX[ly * sz + lz];
-          pz = FX[fyz + ly * sz + lz];
+          xp_x = E[c + plane];
+          xp_y = E[cells + c + plane];
+          xp_z = E[2 * cells + c + plane];
+        } else if (px) {
+          xp_y = value(px, wx0, tag);
+          xp_z = value(px + fyz, wx1, tag);
         } else {
-          py = pz = 0.f;
+          xp_y = xp_z = 0.f;
         }
         float qx, qz, rx, ry;
         if (ly + 1 < vy) {
           qx = E[c + sz];
           qz = E[2 * cells + c + sz];
-        } else if (wyh >= 0) {
-          qx = FY[lx * sz + lz];
-          qz = FY[fxz + lx * sz + lz];
+        } else if (fy) {
+          qx = value(py + lx * sz, wy[0], tag);
+          qz = value(py + fxz + lx * sz, wy[1], tag);
         } else {
           qx = qz = 0.f;
         }
         if (lz + 1 < vz) {
           rx = E[c + 1];
           ry = E[cells + c + 1];
-        } else if (wzh >= 0) {
-          rx = FZ[lx * sy + ly];
-          ry = FZ[fxy + lx * sy + ly];
+        } else if (fz) {
+          rx = value(pz + lx * sy, wz[0], tag);
+          ry = value(pz + fxy + lx * sy, wz[1], tag);
         } else {
           rx = ry = 0.f;
         }
-        const float dx1 = py - ey, dx2 = pz - ez;
+        const float dx1 = xp_y - ey, dx2 = xp_z - ez;
         const float dy0 = qx - ex, dy2 = qz - ez;
         const float dz0 = rx - ex, dz1 = ry - ey;
         const float xa = cH[lx * 3], xb = cH[lx * 3 + 1], xk = cH[lx * 3 + 2];
         float* psi = a.psi_h + i;
         const float dxz = cpml_pre(dx2, xa, xb, xk, cur[0], psi);
         const float dxy = cpml_pre(dx1, xa, xb, xk, cur[1], psi + n);
-        const float dyx = cpml_pre(dy0, ya_h, yb_h, yk_h, cur[2], psi + 2 * n);
-        const float dyz = cpml_pre(dy2, ya_h, yb_h, yk_h, cur[3], psi + 3 * n);
-        const float dzy = cpml_pre(dz1, za_h, zb_h, zk_h, cur[4], psi + 4 * n);
-        const float dzx = cpml_pre(dz0, za_h, zb_h, zk_h, cur[5], psi + 5 * n);
+        const float dyx = cpml_pre(dy0, ya, yb, yk, cur[2], psi + 2 * n);
+        const float dyz = cpml_pre(dy2, ya, yb, yk, cur[3], psi + 3 * n);
+        const float dzy = cpml_pre(dz1, za, zb, zk, cur[4], psi + 4 * n);
+        const float dzx = cpml_pre(dz0, za, zb, zk, cur[5], psi + 5 * n);
         const float ox = H[c], oy = H[cells + c], oz = H[2 * cells + c];
         float hx = ox - S * (dyz - dzy);
         float hy = oy - S * (dzx - dxz);
@@ -481,6 +508,19 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
         H[c] = hx;
         H[cells + c] = hy;
         H[2 * cells + c] = hz;
+        // high H faces for the high neighbours' next E half
+        if (lx == vx - 1) {
+          put(myface + ly * sz + lz, hy, out);
+          put(myface + fyz + ly * sz + lz, hz, out);
+        }
+        if (ly == vy - 1) {
+          put(myface + 2 * fyz + lx * sz + lz, hx, out);
+          put(myface + 2 * fyz + fxz + lx * sz + lz, hz, out);
+        }
+        if (lz == vz - 1) {
+          put(myface + 2 * fyz + 2 * fxz + lx * sy + ly, hx, out);
+          put(myface + 2 * fyz + 2 * fxz + fxy + lx * sy + ly, hy, out);
+        }
         for (int b = 0; b < a.nstage; ++b) {
           const StageBox& st = a.stages[b];
           if (x >= st.x0 && x < st.x1 && y >= st.y0 && y < st.y1 && z >= st.z0 && z < st.z1) {
@@ -499,15 +539,19 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
             d[8 * cnt] = oz;
           }
         }
-        ex = px;
-        ey = py;
-        ez = pz;
+        ex = xp_x;
+        ey = xp_y;
+        ez = xp_z;
+        wy[0] = ny0;
+        wy[1] = ny1;
+        wz[0] = nz0;
+        wz[1] = nz1;
 #pragma unroll
         for (int m = 0; m < 6; ++m) cur[m] = nxt[m];
       }
     }
     __syncthreads();
-    send(myface, H, vx - 1, vy - 1, vz - 1, 2u * k + 3u);  // high H faces
+    if (bail) return;
   }
 #pragma unroll 1
   for (int c = tid; c < cells; c += kThreads) {
