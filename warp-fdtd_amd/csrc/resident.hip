@@ -62,10 +62,10 @@ struct ResArgs {
   int spin_limit;
 };
 
-// LDS floats a sub-box needs: E, H, 1/eps per cell, (a, b, 1/kappa) rows of E
-// and H along each axis, and two lists of z indices (column order).
+// LDS floats a sub-box needs: E, H, 1/eps per cell and (a, b, 1/kappa) rows
+// of E and H along each axis.
 __host__ __device__ inline long lds_floats(int sx, int sy, int sz) {
-  return 7L * sx * sy * sz + 6L * (sx + sy + sz) + 2L * sz;
+  return 7L * sx * sy * sz + 6L * (sx + sy + sz);
 }
 
 // Words of one workgroup's face buffer: H high faces then E low faces, two
@@ -227,32 +227,6 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
   // computed it, right after computing it.  Per-word dependencies order every
   // overwrite after the read of the previous value (see the file header), so
   // the half-steps need no barrier beyond the LDS one between them.
-  // The remaining (interior) columns come z-slab ones first (lz with CPML in
-  // z), so the wavefronts that wait on psi loads are few and the others keep
-  // the SIMDs busy meanwhile.  zl lists, per half, the interior lz values:
-  // [0, ns) in the slab, [ns, vz - 1) outside it.
-  int* zl_e = reinterpret_cast<int*>(co + 6 * R);
-  int* zl_h = zl_e + sz;
-  __shared__ int ns_e, ns_h;
-  if (tid == 0) {
-    int a_ = 0, b_ = 0;
-    for (int q = 1; q < vz; ++q) a_ += cE[(sx + sy + q) * 3] != 0.f;
-    ns_e = a_;
-    int s_ = 0, u_ = a_;
-    for (int q = 1; q < vz; ++q) {
-      if (cE[(sx + sy + q) * 3] != 0.f) zl_e[s_++] = q;
-      else zl_e[u_++] = q;
-    }
-    for (int q = 0; q + 1 < vz; ++q) b_ += cH[(sx + sy + q) * 3] != 0.f;
-    ns_h = b_;
-    s_ = 0;
-    u_ = b_;
-    for (int q = 0; q + 1 < vz; ++q) {
-      if (cH[(sx + sy + q) * 3] != 0.f) zl_h[s_++] = q;
-      else zl_h[u_++] = q;
-    }
-  }
-  __syncthreads();
   const int ncol = vy * vz;
   const bool active = tid < ncol;
   const int nb = vy + vz - 1, w1 = vz - 1;
@@ -267,28 +241,12 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
     ezl = 0;
     hyl = tid - vz;
     hzl = vz - 1;
-  } else if (active) {
-    // interior: (vy - 1) rows x (vz - 1) columns, slab columns of every row first
-    const int m = tid - nb;
-    const int se = (vy - 1) * ns_e, sh = (vy - 1) * ns_h;
-    if (m < se) {
-      eyl = 1 + m / ns_e;
-      ezl = zl_e[m % ns_e];
-    } else {
-      const int r = m - se, w = w1 - ns_e;
-      eyl = 1 + r / w;
-      ezl = zl_e[ns_e + r % w];
-    }
-    if (m < sh) {
-      hyl = m / ns_h;
-      hzl = zl_h[m % ns_h];
-    } else {
-      const int r = m - sh, w = w1 - ns_h;
-      hyl = r / w;
-      hzl = zl_h[ns_h + r % w];
-    }
   } else {
-    eyl = ezl = hyl = hzl = 0;
+    const int m = tid - nb;
+    eyl = w1 > 0 ? 1 + m / w1 : 0;
+    ezl = w1 > 0 ? 1 + m % w1 : 0;
+    hyl = w1 > 0 ? m / w1 : 0;
+    hzl = w1 > 0 ? m % w1 : 0;
   }
   const int plane = sy * sz;
 
