@@ -105,22 +105,64 @@ def measure_resident(solver, layout, steps=64, repeats=5):
     return float(np.median([a.elapsed_time(b) for a, b in events]) * 1e3 / steps)
 
 
-def tune_resident(solver, layout, use_cache=True):
-    """True when the resident kernel beats the tuned fused step on this grid."""
-    key = _key(solver) + f"|resident|{layout.sx}x{layout.sy}x{layout.sz}"
+def candidate_layouts(solver, count=5):
+    """Resident sub-box candidates: for each (sy, sz) the library accepts, its
+    layout with the fewest x planes; the `count` best by (planes, surface)."""
+    nx, ny, nz = solver.shape
+    seen, out = set(), []
+    zs = sorted({v for v in range(4, nz + 1, 4)} | {nz})
+    ys = sorted({v for v in range(2, ny + 1, 2)} | {ny})
+    for sz in zs:
+        for sy in ys:
+            if sy * sz > 1024:
+                break
+            try:
+                L = solver.resident_layout(0, sy, sz)
+            except ValueError:
+                continue
+            key = (L.sx, L.sy, L.sz)
+            if key in seen:
+                continue
+            seen.add(key)
+            out.append(L)
+    out.sort(key=lambda L: (L.sx, L.sx * L.sy + L.sx * L.sz + L.sy * L.sz, L.blocks))
+    return out[:count]
+
+
+def tune_resident(solver, use_cache=True, require=False):
+    """The fastest resident layout if it beats the tuned fused step on this
+    grid (or in any case with ``require``), else None.  Candidates: the
+    library's own choice and the best few (sy, sz) factorisations
+    (candidate_layouts), each timed on the grid."""
+    key = _key(solver) + "|resident-layout"
     if use_cache:
-        if key in _memory:
-            return _memory[key]
-        table = _load()
-        if key in table:
-            _memory[key] = bool(table[key]["resident"])
-            return _memory[key]
+        hit = _memory.get(key)
+        if hit is None:
+            hit = _load().get(key)
+        if hit is not None:
+            _memory[key] = hit
+            if not (hit["resident"] or require):
+                return None
+            try:
+                return solver.resident_layout(*hit["layout"])
+            except ValueError:
+                pass
+    try:
+        layouts = [solver.resident_layout()]
+    except ValueError:
+        return None
+    for L in candidate_layouts(solver):
+        if (L.sx, L.sy, L.sz) != (layouts[0].sx, layouts[0].sy, layouts[0].sz):
+            layouts.append(L)
     fused = measure(solver, solver.x_chunk or 2)
-    resident = measure_resident(solver, layout)
-    best = resident < fused
-    _memory[key] = best
+    timings = {f"{L.sx}x{L.sy}x{L.sz}": measure_resident(solver, L) for L in layouts}
+    best = min(layouts, key=lambda L: timings[f"{L.sx}x{L.sy}x{L.sz}"])
+    best_us = timings[f"{best.sx}x{best.sy}x{best.sz}"]
+    use = best_us < fused
+    entry = {"resident": use, "layout": [best.sx, best.sy, best.sz],
+             "us": {"fused": round(fused, 3), **{k: round(v, 3) for k, v in timings.items()}}}
+    _memory[key] = entry
     table = _load()
-    table[key] = {"resident": best, "us": {"fused": round(fused, 3),
-                                           "resident": round(resident, 3)}}
+    table[key] = entry
     _store(table)
-    return best
+    return best if (use or require) else None

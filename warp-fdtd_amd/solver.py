@@ -477,13 +477,13 @@ class Solver:
             self.x_chunk = tune(self)
         self.resident = False
         if self.resident_mode is True:
-            self.use_resident()
+            if not self.resident_applies():
+                raise ValueError("resident stepping needs full CPML storage, staged monitors "
+                                 "only and no hprev")
+            self.use_resident(tune_resident(self, require=True))
         elif self.resident_mode == "auto" and self.resident_applies():
-            try:
-                layout = self.resident_layout()
-            except ValueError:
-                layout = None
-            if layout is not None and tune_resident(self, layout):
+            layout = tune_resident(self)
+            if layout is not None:
                 self.use_resident(layout)
         return self.x_chunk
 
