@@ -35,7 +35,6 @@ namespace {
 
 constexpr int kThreads = 1024;
 constexpr int kLds = 40896;  // floats of LDS per workgroup (163,584 bytes)
-constexpr int kBatch = 4;    // face words a thread polls per round trip
 
 typedef unsigned long long Word;
 
@@ -115,6 +114,21 @@ __device__ __forceinline__ float cpml_pre(float d, float a, float b, float k, fl
     return k * d + p;
   }
   return k * d;
+}
+
+// An opaque copy: the compiler cannot see through it, so values derived from
+// it inside the step loop are recomputed there instead of being hoisted out
+// of the loop and held in registers for every step (the face addresses of a
+// column would not fit, and their spills cost more than recomputing them).
+template <class T>
+__device__ __forceinline__ T opaque_v(T v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+template <class T>
+__device__ __forceinline__ T opaque_s(T v) {
+  asm volatile("" : "+s"(v));
+  return v;
 }
 
 __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
@@ -277,15 +291,17 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
     // ------------------------------------------------------------ E half
     if (active) {
       const unsigned tag = 2u * k + 1u, out = 2u * k + 2u;
-      const int ly = eyl, lz = ezl, col = ly * sz + lz;
+      Word* const face = opaque_s(a.face);
+      Word* const myface = face + (size_t)wg * per;
+      const int ly = opaque_v(eyl), lz = opaque_v(ezl), col = ly * sz + lz;
       const int y = Y0 + ly, z = Z0 + lz, o = y * nz + z;
       const float ya = cE[(sx + ly) * 3], yb = cE[(sx + ly) * 3 + 1], yk = cE[(sx + ly) * 3 + 2];
       const float za = cE[(sx + sy + lz) * 3], zb = cE[(sx + sy + lz) * 3 + 1],
                   zk = cE[(sx + sy + lz) * 3 + 2];
       const bool fy = ly == 0 && wyl >= 0, fz = lz == 0 && wzl >= 0;
-      const Word* px = wxl >= 0 ? a.face + (size_t)wxl * per + ly * sz + lz : nullptr;
-      const Word* py = fy ? a.face + (size_t)wyl * per + 2 * fyz + lz : nullptr;
-      const Word* pz = fz ? a.face + (size_t)wzl * per + 2 * fyz + 2 * fxz + ly : nullptr;
+      const Word* px = wxl >= 0 ? face + (size_t)wxl * per + ly * sz + lz : nullptr;
+      const Word* py = fy ? face + (size_t)wyl * per + 2 * fyz + lz : nullptr;
+      const Word* pz = fz ? face + (size_t)wzl * per + 2 * fyz + 2 * fxz + ly : nullptr;
       const bool src_here = a.source_x >= X0 && a.source_x < X0 + vx && t < a.steps;
       const float wave_y = src_here ? a.wave_h[2 * a.steps + t] : 0.f;
       const float wave_z = src_here ? a.wave_h[a.steps + t] : 0.f;
@@ -408,15 +424,17 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
     // ------------------------------------------------------------ H half
     if (active) {
       const unsigned tag = 2u * k + 2u, out = 2u * k + 3u;
-      const int ly = hyl, lz = hzl, col = ly * sz + lz;
+      Word* const face = opaque_s(a.face);
+      Word* const myface = face + (size_t)wg * per;
+      const int ly = opaque_v(hyl), lz = opaque_v(hzl), col = ly * sz + lz;
       const int y = Y0 + ly, z = Z0 + lz, o = y * nz + z;
       const float ya = cH[(sx + ly) * 3], yb = cH[(sx + ly) * 3 + 1], yk = cH[(sx + ly) * 3 + 2];
       const float za = cH[(sx + sy + lz) * 3], zb = cH[(sx + sy + lz) * 3 + 1],
                   zk = cH[(sx + sy + lz) * 3 + 2];
       const bool fy = ly == vy - 1 && wyh >= 0, fz = lz == vz - 1 && wzh >= 0;
-      const Word* px = wxh >= 0 ? a.face + (size_t)wxh * per + half + ly * sz + lz : nullptr;
-      const Word* py = fy ? a.face + (size_t)wyh * per + half + 2 * fyz + lz : nullptr;
-      const Word* pz = fz ? a.face + (size_t)wzh * per + half + 2 * fyz + 2 * fxz + ly : nullptr;
+      const Word* px = wxh >= 0 ? face + (size_t)wxh * per + half + ly * sz + lz : nullptr;
+      const Word* py = fy ? face + (size_t)wyh * per + half + 2 * fyz + lz : nullptr;
+      const Word* pz = fz ? face + (size_t)wzh * per + half + 2 * fyz + 2 * fxz + ly : nullptr;
       const bool src_here = a.source_x >= X0 && a.source_x < X0 + vx && t < a.steps;
       const float wave_y = src_here ? a.wave_e[2 * a.steps + t] : 0.f;
       const float wave_z = src_here ? a.wave_e[a.steps + t] : 0.f;
