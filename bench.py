@@ -39,7 +39,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per step measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
 # workload (scaled by a 256 MiB copy: FETCH x2.000, WRITE x1.000), see
 # profiles/hbm_traffic_waveguide_160x96x80.md; None for other shapes.
-MEASURED_TRAFFIC = {(SHAPE, True): 66.14e6, (SHAPE, False): 130.9e6}
+MEASURED_TRAFFIC = {(SHAPE, True): 47.03e6, (SHAPE, False): 130.9e6}
 
 
 def algorithmic_bytes_per_cell(shape, pml):
