@@ -269,11 +269,21 @@ def bench_surrogate(args):
     pred = np.array([sg.predict(net, norm, r)[0] for r in held])
     rel = np.abs(pred[:, f] - Th[:, f]) / np.abs(Th[:, f])
     rho0 = sg.random_designs(1, 100, seed=7)[0]
-    plain = sg.optimize(ev, rho0, steps=args.opt_steps, step=0.05, log=log)
-    sur = sg.optimize(ev, rho0, steps=args.opt_steps, step=0.05, surrogate=(net, norm), log=log)
+    step = args.opt_step
+    plain = sg.optimize(ev, rho0, steps=args.opt_steps, step=step, log=log)
+    sur = sg.optimize(ev, rho0, steps=args.opt_steps, step=step, surrogate=(net, norm), log=log)
+    lag = sg.optimize(ev, rho0, steps=args.opt_steps, step=step, surrogate=(net, norm),
+                      lagged=True, log=log)
     torch.cuda.synchronize()
     within = float(np.mean(rel <= 0.005))
     per_hour = args.opt_steps / sur["seconds"] * 3600
+    # the plain loop's time to the surrogate loop's final T (history[k] is the
+    # T before step k, so k steps reach it)
+    hist_plain = np.array(plain["history"] + [plain["final"]])
+    reach = np.nonzero(hist_plain >= sur["final"])[0]
+    plain_to_same = (float(reach[0]) * plain["seconds"] / args.opt_steps
+                     if len(reach) else None)
+    errors = np.array(sur["errors"])
     _line("designs/hr at <=0.5% T error (surrogate-assisted loop, FDTD-verified)",
           round(per_hour, 1), "designs/hr", args.opt_steps, 0,
           round(sur["seconds"] * 1e3 / args.opt_steps, 3),
@@ -283,11 +293,24 @@ def bench_surrogate(args):
            "train_s": round(train_s, 1), "final_loss": hist[-1],
            "heldout_T_rel_err_median": float(np.median(rel)),
            "heldout_within_0.5pct": within,
+           "heldout_T_range": [float(Th[:, f].min()), float(Th[:, f].max())],
+           "step": step, "loop": "anchored at the last adjoint gradient, Taylor T predictor",
            "plain_final_T": plain["final"], "surrogate_final_T": sur["final"],
-           "final_T_rel_diff": abs(sur["final"] - plain["final"]) / abs(plain["final"]),
            "plain_s": round(plain["seconds"], 2), "surrogate_s": round(sur["seconds"], 2),
            "speedup": round(plain["seconds"] / sur["seconds"], 3),
-           "fallbacks": sur["fallbacks"]}, dtype="bf16")
+           "plain_s_to_surrogate_final_T": None if plain_to_same is None else round(plain_to_same, 2),
+           "speedup_at_same_T": None if plain_to_same is None
+           else round(plain_to_same / sur["seconds"], 3),
+           "fallbacks": sur["fallbacks"],
+           "accepted_surrogate_steps": args.opt_steps - sur["fallbacks"],
+           "proposal_T_rel_err_median": float(np.median(errors)),
+           "proposals_within_0.5pct": float(np.mean(errors <= 0.005)),
+           "proposals_within_1pct": float(np.mean(errors <= 0.01)),
+           "lagged_ablation": {"final_T": lag["final"], "s": round(lag["seconds"], 2),
+                               "fallbacks": lag["fallbacks"]},
+           "plain_forward_adjoint_evals": args.opt_steps,
+           "surrogate_loop_evals": {k: sur["counts"][k] - plain["counts"][k]
+                                    for k in ("forward", "adjoint")}}, dtype="bf16")
 
 
 def main():
@@ -312,6 +335,8 @@ def main():
     p.add_argument("--designs", type=int, default=1024, help="surrogate: training designs")
     p.add_argument("--epochs", type=int, default=40, help="surrogate: training epochs")
     p.add_argument("--opt-steps", type=int, default=200, help="surrogate: optimisation steps")
+    p.add_argument("--opt-step", type=float, default=0.01,
+                   help="surrogate: largest density change of a step")
     args = p.parse_args()
     if args.workload != "waveguide" and args.gpus == 1:
         {"bend-adjoint": bench_adjoint, "batch": bench_batch,

@@ -11,7 +11,9 @@
 * ``optimize`` runs gradient ascent on T at the objective wavelength either
   with the adjoint at every step (``surrogate=None``) or with surrogate
   proposals, each re-verified by a full FDTD forward and replaced by an
-  adjoint step when the surrogate's transmission error exceeds ``tolerance``.
+  adjoint step when the surrogate's transmission error exceeds ``tolerance``
+  (the surrogate's T is corrected by the last verified design's: only the
+  change from it is predicted).
 """
 
 import time
@@ -108,24 +110,56 @@ class Normalizer:
         self.g_scale = float(np.sqrt((G ** 2).mean())) + 1e-12
 
 
+class _Fitter:
+    """Adam steps of the FNO on (rho -> normalised T spectrum, dT/drho)
+    batches held on the GPU (MSE on both heads, the library's kernels)."""
+
+    def __init__(self, net, norm, grad_weight=1.0):
+        from ._lib import lib
+
+        self.net, self.norm, self.grad_weight = net, norm, grad_weight
+        B, dev = net.B, net.device
+        self.d_t = torch.zeros(B, net.F, device=dev)
+        self.d_g = torch.zeros(B, net.S, net.S, device=dev)
+        self.part_t = torch.zeros(int(lib().wfd_mse_partials(B * net.F)), device=dev)
+        self.part_g = torch.zeros(int(lib().wfd_mse_partials(B * net.S * net.S)), device=dev)
+
+    def tensors(self, rhos, T, G):
+        """Device tensors of densities and normalised labels."""
+        dev, norm = self.net.device, self.norm
+        x = torch.from_numpy(np.asarray(rhos, np.float32)).to(dev)
+        tn = torch.from_numpy(((np.asarray(T, np.float64) - norm.t_mean) / norm.t_std)
+                              .astype(np.float32)).to(dev)
+        gn = torch.from_numpy((np.asarray(G, np.float64) / norm.g_scale).astype(np.float32)).to(dev)
+        return x, tn, gn
+
+    def step(self, x, tn, gn, idx, rate):
+        """One Adam step on the rows idx (a device index tensor of net.B rows);
+        returns the batch loss as a device scalar."""
+        from ._lib import call, stream
+
+        net, B = self.net, self.net.B
+        t_out, g_out = net.forward(x.index_select(0, idx).contiguous())
+        tt = tn.index_select(0, idx).contiguous()
+        gt = gn.index_select(0, idx).contiguous()
+        call("wfd_mse", t_out.data_ptr(), tt.data_ptr(), B * net.F, 2.0 / (B * net.F),
+             self.d_t.data_ptr(), self.part_t.data_ptr(), stream())
+        call("wfd_mse", g_out.data_ptr(), gt.data_ptr(), B * net.S * net.S,
+             2.0 * self.grad_weight / (B * net.S * net.S), self.d_g.data_ptr(),
+             self.part_g.data_ptr(), stream())
+        net.backward(self.d_t, self.d_g)
+        net.adam(rate)
+        return (self.part_t.sum() / (B * net.F) + self.grad_weight * self.part_g.sum()
+                / (B * net.S * net.S))
+
+
 def train(net, rhos, T, G, epochs=40, lr=2e-3, grad_weight=1.0, seed=0, norm=None, log=None):
     """Fit the FNO to (rho -> T spectrum, dT/drho) with Adam on the GPU.
     Returns the normalizer and the per-epoch mean loss."""
-    from ._lib import call, lib, stream
-
-    rhos = np.asarray(rhos, np.float32)
-    T = np.asarray(T, np.float64)
-    G = np.asarray(G, np.float64)
-    norm = norm or Normalizer(T, G)
-    dev = net.device
-    x = torch.from_numpy(rhos).to(dev)
-    tn = torch.from_numpy(((T - norm.t_mean) / norm.t_std).astype(np.float32)).to(dev)
-    gn = torch.from_numpy((G / norm.g_scale).astype(np.float32)).to(dev)
+    norm = norm or Normalizer(np.asarray(T, np.float64), np.asarray(G, np.float64))
+    fit = _Fitter(net, norm, grad_weight)
+    x, tn, gn = fit.tensors(rhos, T, G)
     B = net.B
-    d_t = torch.zeros(B, net.F, device=dev)
-    d_g = torch.zeros(B, net.S, net.S, device=dev)
-    part_t = torch.zeros(int(lib().wfd_mse_partials(B * net.F)), device=dev)
-    part_g = torch.zeros(int(lib().wfd_mse_partials(B * net.S * net.S)), device=dev)
     rng = np.random.default_rng(seed)
     history = []
     n = len(rhos)
@@ -134,25 +168,42 @@ def train(net, rhos, T, G, epochs=40, lr=2e-3, grad_weight=1.0, seed=0, norm=Non
         total, batches = 0.0, 0
         rate = lr * 0.5 * (1 + np.cos(np.pi * epoch / max(epochs, 1)))
         for start in range(0, n - B + 1, B):
-            idx = torch.from_numpy(order[start:start + B]).to(dev)
-            t_out, g_out = net.forward(x.index_select(0, idx).contiguous())
-            tt = tn.index_select(0, idx).contiguous()
-            gt = gn.index_select(0, idx).contiguous()
-            call("wfd_mse", t_out.data_ptr(), tt.data_ptr(), B * net.F, 2.0 / (B * net.F),
-                 d_t.data_ptr(), part_t.data_ptr(), stream())
-            call("wfd_mse", g_out.data_ptr(), gt.data_ptr(), B * net.S * net.S,
-                 2.0 * grad_weight / (B * net.S * net.S), d_g.data_ptr(), part_g.data_ptr(),
-                 stream())
-            net.backward(d_t, d_g)
-            net.adam(rate)
-            loss = (part_t.sum() / (B * net.F) + grad_weight * part_g.sum()
-                    / (B * net.S * net.S))
-            total += float(loss)
+            idx = torch.from_numpy(order[start:start + B]).to(net.device)
+            total += float(fit.step(x, tn, gn, idx, rate))
             batches += 1
         history.append(total / max(batches, 1))
         if log:
             log(f"epoch {epoch} loss {history[-1]:.5f}")
     return norm, history
+
+
+class OnlineTuner:
+    """Fine-tuning during the design loop: every adjoint label the loop
+    computes (a fallback) joins a buffer, and ``iters`` Adam steps run on
+    batches of the newest ``recent`` labels topped up with random training
+    designs (so the network adapts to the trajectory without forgetting)."""
+
+    def __init__(self, net, norm, data, iters=8, lr=3e-4, recent=8, seed=0):
+        self.fit = _Fitter(net, norm)
+        self.x, self.tn, self.gn = self.fit.tensors(*data)
+        self.iters, self.lr, self.recent = iters, lr, recent
+        self.rng = np.random.default_rng(seed)
+        self.buf = []
+
+    def add(self, rho, spec, grad):
+        self.buf.append((np.asarray(rho), np.asarray(spec), np.asarray(grad)))
+        net = self.fit.net
+        new = self.buf[-self.recent:]
+        bx, bt, bg = self.fit.tensors([b[0] for b in new], [b[1] for b in new],
+                                      [b[2] for b in new])
+        x = torch.cat([bx, self.x])
+        tn = torch.cat([bt, self.tn])
+        gn = torch.cat([bg, self.gn])
+        k = len(new)
+        for _ in range(self.iters):
+            fill = self.rng.choice(len(self.x), net.B - k, replace=False) + k
+            idx = torch.from_numpy(np.concatenate([np.arange(k), fill])).to(net.device)
+            self.fit.step(x, tn, gn, idx, self.lr)
 
 
 def predict(net, norm, rho):
@@ -171,16 +222,33 @@ def _ascent(rho, g, step):
     return np.clip(rho + step * g / scale, 0.0, 1.0)
 
 
-def optimize(evaluator, rho0, steps=200, step=0.05, surrogate=None, tolerance=0.01, log=None):
+def optimize(evaluator, rho0, steps=200, step=0.05, surrogate=None, tolerance=0.01, log=None,
+             predictor="taylor", require_improvement=True, tuner=None, anchored=True,
+             lagged=False):
     """Gradient ascent on T at the objective wavelength.  ``surrogate`` =
-    (net, norm) proposes each step from the FNO's gradient; every proposal is
-    re-verified by a full FDTD forward, and a proposal whose surrogate T is off
-    by more than ``tolerance`` (relative) is replaced by an adjoint step.
-    Returns a dict with the T trajectory (verified), counts and wall time."""
+    (net, norm) proposes each step; every proposal is re-verified by a full
+    FDTD forward, and a proposal whose surrogate T is off by more than
+    ``tolerance`` (relative) is replaced by an adjoint step.
+
+    Gradients of a proposal step: with ``anchored`` the exact adjoint
+    gradient of the last adjoint design (the anchor) plus the network's
+    change of gradient since it, g_a + g_net(rho) - g_net(rho_a) (the first
+    step takes an adjoint step to get an anchor); else the network's g_net.
+    ``lagged`` drops the network's correction (g_a alone: the ablation
+    without a surrogate).  The surrogate's T of a proposal (``predictor``),
+    with T0 the last verified T and d = proposal - current design:
+    "taylor": T0 + <g(current) + g(proposal), d> / 2 (trapezoid rule);
+    "difference": T0 + net(proposal) - net(current) with the network's T
+    head; "absolute": net(proposal).  ``require_improvement``: a proposal
+    whose verified T is below the current one is replaced by an adjoint step
+    as well, so accepted surrogate steps never lose ground.  ``tuner`` (an
+    OnlineTuner) learns from every adjoint label the loop computes.
+    Returns a dict with the T trajectory (verified), the surrogate's relative
+    T error on every proposal, counts and wall time."""
     f = objective_index()
     rho = np.array(rho0, np.float64)
     start = time.perf_counter()
-    history, fallbacks = [], 0
+    history, errors, fallbacks = [], [], 0
     if surrogate is None:
         for k in range(steps):
             spec, g = evaluator.adjoint(rho)
@@ -191,22 +259,55 @@ def optimize(evaluator, rho0, steps=200, step=0.05, surrogate=None, tolerance=0.
         final = float(evaluator.transmission(rho)[f])
     else:
         net, norm = surrogate
+        anchor = None  # (network gradient at the anchor, adjoint gradient there)
+
+        def gradient(pred):
+            if not anchored:
+                return pred[1]
+            if lagged:
+                return anchor[1]
+            return anchor[1] + pred[1] - anchor[0]
+
+        def adjoint_step(rho):
+            nonlocal anchor
+            spec, g = evaluator.adjoint(rho)
+            if tuner is not None:
+                tuner.add(rho, spec, g)
+            anchor = (predict(net, norm, rho)[1], g)
+            return _ascent(rho, g, step)
+
         current = float(evaluator.transmission(rho)[f])
+        here = predict(net, norm, rho)
         for _ in range(steps):
-            _, g_hat = predict(net, norm, rho)
-            proposal = _ascent(rho, g_hat, step)
-            t_hat = predict(net, norm, proposal)[0][f]
-            t_true = float(evaluator.transmission(proposal)[f])
-            if abs(t_hat - t_true) > tolerance * max(abs(t_true), 1e-12):
+            if anchored and anchor is None:
                 fallbacks += 1
-                spec, g = evaluator.adjoint(rho)
-                proposal = _ascent(rho, g, step)
+                proposal = adjoint_step(rho)
                 t_true = float(evaluator.transmission(proposal)[f])
-            rho, current = proposal, t_true
+                there = predict(net, norm, proposal)
+            else:
+                g_here = gradient(here)
+                proposal = _ascent(rho, g_here, step)
+                there = predict(net, norm, proposal)
+                d = proposal - rho
+                if predictor == "taylor":
+                    t_hat = current + 0.5 * float(((g_here + gradient(there)) * d).sum())
+                elif predictor == "difference":
+                    t_hat = current + there[0][f] - here[0][f]
+                else:
+                    t_hat = there[0][f]
+                t_true = float(evaluator.transmission(proposal)[f])
+                errors.append(abs(t_hat - t_true) / max(abs(t_true), 1e-12))
+                if errors[-1] > tolerance or (require_improvement and t_true < current):
+                    fallbacks += 1
+                    proposal = adjoint_step(rho)
+                    t_true = float(evaluator.transmission(proposal)[f])
+                    there = predict(net, norm, proposal)
+            rho, current, here = proposal, t_true, there
             history.append(current)
             if log and len(history) % 20 == 1:
                 log(f"surrogate step {len(history) - 1}: T = {current:.5f}, fallbacks {fallbacks}")
         final = current
     torch.cuda.synchronize()
     return {"final": final, "history": history, "rho": rho, "fallbacks": fallbacks,
-            "seconds": time.perf_counter() - start, "counts": dict(evaluator.counts)}
+            "errors": errors, "seconds": time.perf_counter() - start,
+            "counts": dict(evaluator.counts)}
