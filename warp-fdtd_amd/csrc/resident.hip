@@ -105,12 +105,30 @@ __device__ __forceinline__ bool take(const ResArgs& a, const Word* p, unsigned t
   }
 }
 
-// One CPML partial with the psi value already loaded (kernels.py:29-47).
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+
+// A raw buffer over `bytes` bytes at p: loads past the end read 0 and stores
+// past it are dropped.  Through it a psi component is addressed by a 32-bit
+// lane offset plus a workgroup-uniform component offset (an SGPR), instead of
+// a 64-bit address per lane and component.
+__device__ __forceinline__ Rsrc buffer(const float* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), 0, (int)bytes, 0x00020000);
+}
+// (the b32 builtins move 32-bit integers: the floats travel as their bits)
+__device__ __forceinline__ float ld(Rsrc r, int voff, int soff) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void st(Rsrc r, float v, int voff, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
+}
+
+// One CPML partial with the psi value already loaded (kernels.py:29-47); the
+// new psi goes to byte offset voff + soff of the buffer.
 __device__ __forceinline__ float cpml_pre(float d, float a, float b, float k, float psi_old,
-                                          float* __restrict__ psi_at) {
+                                          Rsrc r, int voff, int soff) {
   if (a != 0.f) {
     const float p = b * psi_old + a * d;
-    *psi_at = p;
+    st(r, p, voff, soff);
     return k * d + p;
   }
   return k * d;
@@ -265,17 +283,20 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
   const int plane = sy * sz;
 
   // psi of one plane of a column (zero where the axis has no CPML there)
-  auto load_psi = [&](const float* psi, const float* cf, float ya, float za, int i, int lx,
+  // psi component k of cell i sits at byte i * 4 + k * pb of the buffers
+  const int pb = n * 4;
+  const Rsrc rpe = buffer(a.psi_e, 6u * (unsigned)pb), rph = buffer(a.psi_h, 6u * (unsigned)pb);
+  auto load_psi = [&](Rsrc r, const float* cf, float ya, float za, int i, int lx,
                       float (&p)[6]) {
     const bool ax = cf[lx * 3] != 0.f;
-    p[0] = ax ? psi[i] : 0.f;
-    p[1] = ax ? psi[n + i] : 0.f;
-    p[2] = ya != 0.f ? psi[2 * n + i] : 0.f;
-    p[3] = ya != 0.f ? psi[3 * n + i] : 0.f;
-    p[4] = za != 0.f ? psi[4 * n + i] : 0.f;
-    p[5] = za != 0.f ? psi[5 * n + i] : 0.f;
-  };
-  // a face value: the word already loaded if it carries the tag, else poll
+    const int v = i * 4;
+    p[0] = ax ? ld(r, v, 0) : 0.f;
+    p[1] = ax ? ld(r, v, pb) : 0.f;
+    p[2] = ya != 0.f ? ld(r, v, 2 * pb) : 0.f;
+    p[3] = ya != 0.f ? ld(r, v, 3 * pb) : 0.f;
+    p[4] = za != 0.f ? ld(r, v, 4 * pb) : 0.f;
+    p[5] = za != 0.f ? ld(r, v, 5 * pb) : 0.f;
+  };  // a face value: the word already loaded if it carries the tag, else poll
   auto value = [&](const Word* p, Word w, unsigned tag) {
     if ((unsigned)(w >> 32) == tag) return __uint_as_float((unsigned)w);
     float v = 0.f;
@@ -310,7 +331,7 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
       int lx = vx > 1 ? 1 : 0;
       float cur[6], nxt[6];
       Word wy[2] = {0, 0}, wz[2] = {0, 0};
-      load_psi(a.psi_e, cE, ya, za, (X0 + lx) * P + o, lx, cur);
+      load_psi(rpe, cE, ya, za, (X0 + lx) * P + o, lx, cur);
       if (fy) {
         wy[0] = peek(py + lx * sz);
         wy[1] = peek(py + fxz + lx * sz);
@@ -329,7 +350,7 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
         const int nlx = q + 2 < vx ? lx + 1 : 0;  // next plane of the march
         Word ny0 = 0, ny1 = 0, nz0 = 0, nz1 = 0;
         if (q + 1 < vx) {
-          load_psi(a.psi_e, cE, ya, za, (X0 + nlx) * P + o, nlx, nxt);
+          load_psi(rpe, cE, ya, za, (X0 + nlx) * P + o, nlx, nxt);
           if (fy) {
             ny0 = peek(py + nlx * sz);
             ny1 = peek(py + fxz + nlx * sz);
@@ -376,13 +397,13 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
         const float dy0 = hx - yx, dy2 = hz - yz;
         const float dz0 = hx - zx, dz1 = hy - zy;
         const float xa = cE[lx * 3], xb = cE[lx * 3 + 1], xk = cE[lx * 3 + 2];
-        float* psi = a.psi_e + i;
-        const float dxz = cpml_pre(dx2, xa, xb, xk, cur[0], psi);
-        const float dxy = cpml_pre(dx1, xa, xb, xk, cur[1], psi + n);
-        const float dyx = cpml_pre(dy0, ya, yb, yk, cur[2], psi + 2 * n);
-        const float dyz = cpml_pre(dy2, ya, yb, yk, cur[3], psi + 3 * n);
-        const float dzy = cpml_pre(dz1, za, zb, zk, cur[4], psi + 4 * n);
-        const float dzx = cpml_pre(dz0, za, zb, zk, cur[5], psi + 5 * n);
+        const int v = i * 4;
+        const float dxz = cpml_pre(dx2, xa, xb, xk, cur[0], rpe, v, 0);
+        const float dxy = cpml_pre(dx1, xa, xb, xk, cur[1], rpe, v, pb);
+        const float dyx = cpml_pre(dy0, ya, yb, yk, cur[2], rpe, v, 2 * pb);
+        const float dyz = cpml_pre(dy2, ya, yb, yk, cur[3], rpe, v, 3 * pb);
+        const float dzy = cpml_pre(dz1, za, zb, zk, cur[4], rpe, v, 4 * pb);
+        const float dzx = cpml_pre(dz0, za, zb, zk, cur[5], rpe, v, 5 * pb);
         const float ie = IE[c];
         float ex = E[c] + (S * (dyz - dzy)) * ie;
         float ey = E[cells + c] + (S * (dzx - dxz)) * ie;
@@ -443,7 +464,7 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
       float ex = E[col], ey = E[cells + col], ez = E[2 * cells + col];
       float cur[6], nxt[6];
       Word wy[2] = {0, 0}, wz[2] = {0, 0};
-      load_psi(a.psi_h, cH, ya, za, X0 * P + o, 0, cur);
+      load_psi(rph, cH, ya, za, X0 * P + o, 0, cur);
       if (fy) {
         wy[0] = peek(py);
         wy[1] = peek(py + fxz);
@@ -461,7 +482,7 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
       for (int lx = 0; lx < vx; ++lx) {
         Word ny0 = 0, ny1 = 0, nz0 = 0, nz1 = 0;
         if (lx + 1 < vx) {
-          load_psi(a.psi_h, cH, ya, za, (X0 + lx + 1) * P + o, lx + 1, nxt);
+          load_psi(rph, cH, ya, za, (X0 + lx + 1) * P + o, lx + 1, nxt);
           if (fy) {
             ny0 = peek(py + (lx + 1) * sz);
             ny1 = peek(py + fxz + (lx + 1) * sz);
@@ -508,13 +529,13 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
         const float dy0 = qx - ex, dy2 = qz - ez;
         const float dz0 = rx - ex, dz1 = ry - ey;
         const float xa = cH[lx * 3], xb = cH[lx * 3 + 1], xk = cH[lx * 3 + 2];
-        float* psi = a.psi_h + i;
-        const float dxz = cpml_pre(dx2, xa, xb, xk, cur[0], psi);
-        const float dxy = cpml_pre(dx1, xa, xb, xk, cur[1], psi + n);
-        const float dyx = cpml_pre(dy0, ya, yb, yk, cur[2], psi + 2 * n);
-        const float dyz = cpml_pre(dy2, ya, yb, yk, cur[3], psi + 3 * n);
-        const float dzy = cpml_pre(dz1, za, zb, zk, cur[4], psi + 4 * n);
-        const float dzx = cpml_pre(dz0, za, zb, zk, cur[5], psi + 5 * n);
+        const int v = i * 4;
+        const float dxz = cpml_pre(dx2, xa, xb, xk, cur[0], rph, v, 0);
+        const float dxy = cpml_pre(dx1, xa, xb, xk, cur[1], rph, v, pb);
+        const float dyx = cpml_pre(dy0, ya, yb, yk, cur[2], rph, v, 2 * pb);
+        const float dyz = cpml_pre(dy2, ya, yb, yk, cur[3], rph, v, 3 * pb);
+        const float dzy = cpml_pre(dz1, za, zb, zk, cur[4], rph, v, 4 * pb);
+        const float dzx = cpml_pre(dz0, za, zb, zk, cur[5], rph, v, 5 * pb);
         const float ox = H[c], oy = H[cells + c], oz = H[2 * cells + c];
         float hx = ox - S * (dyz - dzy);
         float hy = oy - S * (dzx - dxz);
@@ -699,6 +720,9 @@ int wfd_resident_run(const wfd_yee_plan* p, const wfd_resident_layout* L, void* 
               "resident_run: null field, psi, coeff or counter pointer");
   WFD_REQUIRE(p->nx > 0 && p->ny > 0 && p->nz > 0 && cells(p->nx, p->ny, p->nz) <= kMaxCells,
               "resident_run: bad grid");
+  // psi is addressed through 32-bit buffer offsets (6 components of 4 bytes)
+  WFD_REQUIRE(cells(p->nx, p->ny, p->nz) * 24 < (int64_t(1) << 31),
+              "resident_run: grid too large for 32-bit psi offsets");
   WFD_REQUIRE(p->coeff_rows >= p->nx && p->coeff_rows >= p->ny && p->coeff_rows >= p->nz,
               "resident_run: coeff_rows < max(shape)");
   WFD_REQUIRE(p->source_x < p->nx, "resident_run: source plane outside the grid");
