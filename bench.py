@@ -320,6 +320,9 @@ def main():
     p.add_argument("--warmup", type=int, default=256)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle sample")
     p.add_argument("--kernel-repeats", type=int, default=200)
+    p.add_argument("--graph-steps", type=int, default=256,
+                   help="steps per captured graph / resident launch (and detector stage slots; "
+                        "the library default is the reference's 64)")
     p.add_argument("--x-chunk", type=int, default=0,
                    help="planes per thread column of the fused step (0: autotune)")
     p.add_argument("--resident", choices=("auto", "on", "off"), default="auto",
@@ -355,7 +358,8 @@ def main():
 
     pkg, case = build_case()
     resident = {"auto": "auto", "on": True, "off": False}[args.resident]
-    s = pkg.solver.Solver(case, x_chunk=args.x_chunk, resident=resident)
+    s = pkg.solver.Solver(case, x_chunk=args.x_chunk, resident=resident,
+                          graph_steps=args.graph_steps)
     pkg.detectors.attach(s)
     s.compile()
     if args.warmup:
@@ -378,16 +382,16 @@ def main():
     finite = bool(np.isfinite(s.E.numpy()).all() and np.isfinite(s.H.numpy()).all())
 
     B, f_cpml = algorithmic_bytes_per_cell(s.shape, PML)
-    median_s, mean_s = kernel_time_per_step(pkg, s, args.kernel_repeats)
+    median_s, mean_s = kernel_time_per_step(pkg, s, args.kernel_repeats, chunk=s.graph_steps)
     achieved = cells * B / mean_s / 1e9
     traffic = MEASURED_TRAFFIC.get((tuple(s.shape), bool(s.resident)))
-    launch_steps = 64 if s.resident else 1
+    launch_steps = s.graph_steps if s.resident else 1
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": None if traffic is None else round(traffic * launch_steps),
                 "traffic_per_step_bytes": traffic,
                 "algorithmic_bytes_per_step": round(cells * B),
-                "kernel": ("k_resident (steps per launch: 64)" if s.resident
+                "kernel": (f"k_resident (steps per launch: {s.graph_steps})" if s.resident
                            else "k_yee_e + k_yee_h (one fused step)"),
                 "bytes_per_cell_update": round(B, 2), "f_cpml": round(f_cpml, 4),
                 "kernel_us_per_step_mean": round(mean_s * 1e6, 3),

@@ -11,6 +11,9 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kFreqGroup = 8;
+constexpr int kTile = 64;  // steps of DFT weights staged in LDS at a time
+constexpr int kSeg = 8;    // step segments per (component, cell) of a DFT plane
+constexpr int kPairs = kBlock / kSeg;
 
 struct Replay {
   const float* stage;
@@ -153,31 +156,47 @@ __global__ __launch_bounds__(kBlock) void k_replay(ReplayArgs args, int slots,
     for (int c = 0; c < 6; ++c) ph[c * r.count + i] = acc[c];
     return;
   }
-  // kind 2: FP64 multi-frequency DFT; one thread per (component, cell), frequency
-  // groups of 8 along the block index.
+  // kind 2: FP64 multi-frequency DFT.  A block holds kPairs (component, cell)
+  // pairs times kSeg step segments: thread (seg, pair) sums the chunk's steps
+  // t with (t - t_lo) % kSeg == seg, in order, for kFreqGroup frequencies
+  // (frequency groups along the block index); the segments' sums are then
+  // added in segment order and the total added to the phasor.  The weights of
+  // the block's frequencies are staged in LDS one tile of kTile steps at a
+  // time (every thread of the block reads the same ones).
+  __shared__ double wsh[kTile * kFreqGroup * 2];
+  __shared__ double part[kSeg][kFreqGroup * 2][kPairs];
   const int groups = (r.nf + kFreqGroup - 1) / kFreqGroup;
-  const int per_group = (6 * r.count + kBlock - 1) / kBlock;
+  const int per_group = (6 * r.count + kPairs - 1) / kPairs;
   const int g = local / per_group;
-  const int ci = (local - g * per_group) * kBlock + threadIdx.x;
-  if (g >= groups || ci >= 6 * r.count) return;
-  const int c = ci / r.count, i = ci - c * r.count;
-  const int f0 = g * kFreqGroup;
+  const int seg = threadIdx.x / kPairs, lane = threadIdx.x % kPairs;
+  const int ci = (local - g * per_group) * kPairs + lane;
+  const bool valid = g < groups && ci < 6 * r.count;  // block-uniform g; ci per thread
+  const int c = valid ? ci / r.count : 0, i = valid ? ci - c * r.count : 0;
+  const int f0 = g * kFreqGroup, fn = min(kFreqGroup, r.nf - f0);
   int x, y, z;
   plane_xyz(r, i, x, y, z);
   double re[kFreqGroup], im[kFreqGroup];
 #pragma unroll
   for (int q = 0; q < kFreqGroup; ++q) re[q] = im[q] = 0.0;
-  // steps t in [t_lo, t_hi]: the chunk's steps inside the recording window;
-  // unrolled by 4 so the colocation loads of four steps are in flight at once
+  // steps t in [t_lo, t_hi]: the chunk's steps inside the recording window
   const int t_lo = max(last - m + 1, 0), t_hi = min(last, r.steps - 1);
-#pragma unroll 4
-  for (int t = t_lo; t <= t_hi; ++t) {
-    s.base = r.stage + (size_t)(t % slots) * 9 * r.fcount;
-    const double v = (double)colocated(s, c, x, y, z, r.exact);
-    const double* w = r.weights64 + 2 * ((size_t)t * r.nf + f0);
+  for (int t0 = t_lo; t0 <= t_hi; t0 += kTile) {
+    const int len = min(kTile, t_hi - t0 + 1);
+    __syncthreads();
+    for (int e = threadIdx.x; e < len * kFreqGroup * 2; e += kBlock) {
+      const int tt = e / (kFreqGroup * 2), q = (e / 2) % kFreqGroup, half = e % 2;
+      wsh[e] = q < fn ? r.weights64[2 * ((size_t)(t0 + tt) * r.nf + f0 + q) + half] : 0.0;
+    }
+    __syncthreads();
+    if (!valid) continue;
+    // unrolled by 2 so the colocation loads of two steps are in flight at once
+#pragma unroll 2
+    for (int tt = seg; tt < len; tt += kSeg) {
+      s.base = r.stage + (size_t)((t0 + tt) % slots) * 9 * r.fcount;
+      const double v = (double)colocated(s, c, x, y, z, r.exact);
+      const double* w = wsh + tt * kFreqGroup * 2;
 #pragma unroll
-    for (int q = 0; q < kFreqGroup; ++q) {
-      if (f0 + q < r.nf) {
+      for (int q = 0; q < kFreqGroup; ++q) {
         re[q] += w[2 * q] * v;
         im[q] += w[2 * q + 1] * v;
       }
@@ -185,11 +204,24 @@ __global__ __launch_bounds__(kBlock) void k_replay(ReplayArgs args, int slots,
   }
 #pragma unroll
   for (int q = 0; q < kFreqGroup; ++q) {
-    if (f0 + q < r.nf) {
+    part[seg][2 * q][lane] = re[q];
+    part[seg][2 * q + 1][lane] = im[q];
+  }
+  __syncthreads();
+  if (!valid || seg != 0) return;
+#pragma unroll
+  for (int q = 0; q < kFreqGroup; ++q) {
+    if (q < fn) {
+      double sr = part[0][2 * q][lane], si = part[0][2 * q + 1][lane];
+#pragma unroll
+      for (int k2 = 1; k2 < kSeg; ++k2) {
+        sr += part[k2][2 * q][lane];
+        si += part[k2][2 * q + 1][lane];
+      }
       double2* p = reinterpret_cast<double2*>(r.phasor64) + ((size_t)(f0 + q) * 6 * r.count + ci);
       double2 cur = *p;
-      cur.x += re[q];
-      cur.y += im[q];
+      cur.x += sr;
+      cur.y += si;
       *p = cur;
     }
   }
@@ -262,7 +294,7 @@ int wfd_replay_monitors(const wfd_replay* monitors, int count, int slots, const 
     args.begin[k] = blocks;
     if (d.kind == 1) blocks += 1;
     else if (d.kind == 0) blocks += (r.count + kBlock - 1) / kBlock;
-    else blocks += ((6 * r.count + kBlock - 1) / kBlock) * ((d.nf + kFreqGroup - 1) / kFreqGroup);
+    else blocks += ((6 * r.count + kPairs - 1) / kPairs) * ((d.nf + kFreqGroup - 1) / kFreqGroup);
   }
   args.begin[count] = blocks;
   hipLaunchKernelGGL(k_replay, dim3(blocks), dim3(kBlock), 0, as_stream(stream), args, slots,
