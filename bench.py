@@ -38,8 +38,10 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per step measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
 # workload (scaled by a 256 MiB copy: FETCH x2.000, WRITE x1.000), see
-# profiles/hbm_traffic_waveguide_160x96x80.md; None for other shapes.
-MEASURED_TRAFFIC = {(SHAPE, True): 47.03e6, (SHAPE, False): 130.9e6}
+# profiles/hbm_traffic_waveguide_160x96x80.md; keyed by (shape, resident,
+# steps per launch); None for others.
+MEASURED_TRAFFIC = {(SHAPE, True, 64): 47.03e6, (SHAPE, True, 256): 46.13e6,
+                    (SHAPE, False, 1): 130.9e6}
 
 
 def algorithmic_bytes_per_cell(shape, pml):
@@ -384,8 +386,8 @@ def main():
     B, f_cpml = algorithmic_bytes_per_cell(s.shape, PML)
     median_s, mean_s = kernel_time_per_step(pkg, s, args.kernel_repeats, chunk=s.graph_steps)
     achieved = cells * B / mean_s / 1e9
-    traffic = MEASURED_TRAFFIC.get((tuple(s.shape), bool(s.resident)))
     launch_steps = s.graph_steps if s.resident else 1
+    traffic = MEASURED_TRAFFIC.get((tuple(s.shape), bool(s.resident), launch_steps))
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": None if traffic is None else round(traffic * launch_steps),
