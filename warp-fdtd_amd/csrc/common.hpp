@@ -52,6 +52,22 @@ struct F3 {
   float x, y, z;
 };
 
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+
+// A raw buffer over `bytes` bytes at p: loads past the end read 0 and stores
+// past it are dropped.  Addressed by a 32-bit lane offset plus a uniform
+// offset (an SGPR) instead of a 64-bit address per lane.
+__device__ __forceinline__ Rsrc buffer(const float* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), 0, (int)bytes, 0x00020000);
+}
+// (the b32 builtins move 32-bit integers: the floats travel as their bits)
+__device__ __forceinline__ float ld(Rsrc r, int voff, int soff) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void st(Rsrc r, float v, int voff, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
+}
+
 // Device copy of a wfd_stage footprint (box plus its cell count).
 struct StageBox {
   float* buf;

@@ -85,6 +85,79 @@ __device__ __forceinline__ void plane_xyz(const Replay& r, int i, int& x, int& y
   z = r.z0 + i % r.sz;
 }
 
+// The staged values one colocated component reads, as byte offsets into a
+// step's slot (4 * (field * fcount + footprint index); past the slot's end
+// for points outside the grid, which a buffer load reads as 0), in the order
+// `colocated` consumes them, so a loop over steps only loads and evaluates
+// the same expression: form 0 one value, 1 the four-point E average, 2 the
+// two-point H average of (H + H_old) / 2, 3 the eight-point one.
+struct Terms {
+  int off[16];
+  int form;
+};
+
+constexpr int kOutside = 0x7ffffff0;  // beyond any slot: reads 0
+
+__device__ __forceinline__ Terms make_terms(const Replay& r, int c, int x, int y, int z,
+                                            int nx, int ny, int nz) {
+  Terms k;
+  int n = 0;
+  auto add = [&](int f, int px, int py, int pz) {
+    const bool in = px >= 0 && py >= 0 && pz >= 0 && px < nx && py < ny && pz < nz;
+    k.off[n++] =
+        in ? 4 * (f * r.fcount + ((px - r.fx0) * r.fsy + (py - r.fy0)) * r.fsz + (pz - r.fz0))
+           : kOutside;
+  };
+  for (int j = 0; j < 16; ++j) k.off[j] = kOutside;
+  if (!r.exact) {
+    k.form = 0;
+    add(c, x, y, z);
+  } else if (c == 0) {
+    k.form = 1;
+    add(0, x, y, z), add(0, x - 1, y, z), add(0, x, y, z + 1), add(0, x - 1, y, z + 1);
+  } else if (c == 1) {
+    k.form = 1;
+    add(1, x, y, z), add(1, x, y - 1, z), add(1, x, y, z + 1), add(1, x, y - 1, z + 1);
+  } else if (c == 2) {
+    k.form = 0;
+    add(2, x, y, z);
+  } else if (c == 3) {
+    k.form = 2;
+    add(3, x, y, z), add(6, x, y, z), add(3, x, y - 1, z), add(6, x, y - 1, z);
+  } else if (c == 4) {
+    k.form = 2;
+    add(4, x, y, z), add(7, x, y, z), add(4, x - 1, y, z), add(7, x - 1, y, z);
+  } else {
+    k.form = 3;
+    const int px[8] = {x, x - 1, x, x - 1, x, x - 1, x, x - 1};
+    const int py[8] = {y, y, y - 1, y - 1, y, y, y - 1, y - 1};
+    const int pz[8] = {z, z, z, z, z + 1, z + 1, z + 1, z + 1};
+    for (int j = 0; j < 8; ++j) add(5, px[j], py[j], pz[j]), add(8, px[j], py[j], pz[j]);
+  }
+  return k;
+}
+
+// colocated() of one step through precomputed terms: the same operations in
+// the same order.  `ring` is a buffer over the whole stage ring and `so` the
+// step's slot offset in bytes (an outside term stays past the ring's end).
+__device__ __forceinline__ float eval_terms(const Terms& k, Rsrc ring, unsigned so) {
+  auto v = [&](int j) { return ld(ring, (int)((unsigned)k.off[j] + so), 0); };
+  switch (k.form) {
+    case 0:
+      return v(0);
+    case 1:
+      return 0.25f * (((v(0) + v(1)) + v(2)) + v(3));
+    case 2:
+      return 0.5f * ((0.5f * (v(0) + v(1))) + (0.5f * (v(2) + v(3))));
+    default: {
+      float t[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) t[j] = 0.5f * (v(2 * j) + v(2 * j + 1));
+      return 0.125f * (((((((t[0] + t[1]) + t[2]) + t[3]) + t[4]) + t[5]) + t[6]) + t[7]);
+    }
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_replay(ReplayArgs args, int slots,
                                                    const int* __restrict__ step, int m,
                                                    int nx, int ny, int nz) {
@@ -175,6 +248,9 @@ __global__ __launch_bounds__(kBlock) void k_replay(ReplayArgs args, int slots,
   const int f0 = g * kFreqGroup, fn = min(kFreqGroup, r.nf - f0);
   int x, y, z;
   plane_xyz(r, i, x, y, z);
+  const Terms terms = make_terms(r, c, x, y, z, nx, ny, nz);
+  const unsigned slot_bytes = 36u * (unsigned)r.fcount;  // 9 fields of a step
+  const Rsrc ring = buffer(r.stage, slot_bytes * (unsigned)slots);
   double re[kFreqGroup], im[kFreqGroup];
 #pragma unroll
   for (int q = 0; q < kFreqGroup; ++q) re[q] = im[q] = 0.0;
@@ -182,6 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_replay(ReplayArgs args, int slots,
   const int t_lo = max(last - m + 1, 0), t_hi = min(last, r.steps - 1);
   for (int t0 = t_lo; t0 <= t_hi; t0 += kTile) {
     const int len = min(kTile, t_hi - t0 + 1);
+    const int s0 = t0 % slots;  // len <= m <= slots: a tile wraps the ring at most once
     __syncthreads();
     for (int e = threadIdx.x; e < len * kFreqGroup * 2; e += kBlock) {
       const int tt = e / (kFreqGroup * 2), q = (e / 2) % kFreqGroup, half = e % 2;
@@ -192,8 +269,8 @@ __global__ __launch_bounds__(kBlock) void k_replay(ReplayArgs args, int slots,
     // unrolled by 2 so the colocation loads of two steps are in flight at once
 #pragma unroll 2
     for (int tt = seg; tt < len; tt += kSeg) {
-      s.base = r.stage + (size_t)((t0 + tt) % slots) * 9 * r.fcount;
-      const double v = (double)colocated(s, c, x, y, z, r.exact);
+      const int slot = s0 + tt < slots ? s0 + tt : s0 + tt - slots;  // (t0 + tt) % slots
+      const double v = (double)eval_terms(terms, ring, (unsigned)slot * slot_bytes);
       const double* w = wsh + tt * kFreqGroup * 2;
 #pragma unroll
       for (int q = 0; q < kFreqGroup; ++q) {
@@ -259,6 +336,9 @@ int wfd_replay_monitors(const wfd_replay* monitors, int count, int slots, const 
                     f[1] >= d.x0 + d.sx && f[3] >= d.y0 + d.sy && f[5] >= (hiz > nz ? nz : hiz),
                 "replay: detector %d footprint does not cover its stencil", k);
     WFD_REQUIRE(d.kind >= 0 && d.kind <= 2, "replay: detector %d kind", k);
+    WFD_REQUIRE((int64_t)36 * (f[1] - f[0]) * (f[3] - f[2]) * (f[5] - f[4]) * slots <
+                    (int64_t)kOutside - 16,
+                "replay: detector %d stage ring exceeds 32-bit offsets", k);
     WFD_REQUIRE(d.kind == 2 || d.indices, "replay: detector %d without indices", k);
     WFD_REQUIRE(d.kind != 0 || (d.weights && d.phasor), "replay: phasor detector %d arrays", k);
     WFD_REQUIRE(d.kind != 1 || (d.flux && d.area && d.axis >= 0 && d.axis < 3),

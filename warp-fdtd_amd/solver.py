@@ -422,14 +422,22 @@ class Solver:
             raise RuntimeError("resident stepping timed out waiting for a neighbouring "
                                "workgroup; the fields of this run are invalid")
 
-    def resident_steps(self, count, record=True, spin_limit=0):
+    def resident_steps(self, count, record=True, spin_limit=0, events=None):
         """`count` steps as one cooperative launch; staged monitors (record)
-        replay afterwards, so count must not exceed graph_steps then."""
+        replay afterwards, so count must not exceed graph_steps then.
+        `events` (a list): a (start, end, count) tuple of timing events
+        recorded on the launch stream around the launch is appended."""
         if record and self.monitors and count > self.graph_steps:
             raise ValueError("recorded resident chunks are at most graph_steps long")
         plan = self._plan(stage=record)
+        if events is not None:
+            pair = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            pair[0].record()
         call("wfd_resident_run", ctypes.byref(plan), ctypes.byref(self.layout),
              ptr(self._workspace), int(count), int(spin_limit), stream())
+        if events is not None:
+            pair[1].record()
+            events.append((pair[0], pair[1], int(count)))
         if record:
             self._replay_staged(count)
 
@@ -551,17 +559,18 @@ class Solver:
             t += length
         torch.cuda.synchronize(self.device)
 
-    def run_steps(self, count, synchronize=True):
+    def run_steps(self, count, synchronize=True, events=None):
         """Advance exactly `count` steps from the current step with graph
         replays (graphs of missing lengths are captured on first use), or
-        resident launches of at most graph_steps steps."""
+        resident launches of at most graph_steps steps (`events`: see
+        resident_steps)."""
         if not self._compiled:
             raise RuntimeError("compile() must run before run_steps()")
         if self.resident:
             left = int(count)
             while left > 0:
                 length = min(self.graph_steps, left) if self.monitors else left
-                self.resident_steps(length, record=bool(self.monitors))
+                self.resident_steps(length, record=bool(self.monitors), events=events)
                 left -= length
             if synchronize:
                 self.check()
