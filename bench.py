@@ -144,6 +144,26 @@ def copy_probe(mib):
     print(json.dumps({"copy_mib": mib, "copies": 8}), flush=True)
 
 
+def copy_ceiling(mib=1024, repeats=10):
+    """Measured device-copy bandwidth in GB/s (read + write bytes / time) of
+    `mib` MiB buffers, larger than the 256 MB Infinity Cache: the practical
+    HBM ceiling beside the 8 TB/s peak."""
+    import torch
+
+    a = torch.ones(mib * 2 ** 18, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(repeats):
+        b.copy_(a)
+    end.record()
+    torch.cuda.synchronize()
+    seconds = start.elapsed_time(end) * 1e-3 / repeats
+    del a, b
+    return 2 * mib * 2 ** 20 / seconds / 1e9
+
+
 def log(message):
     """Progress line on stderr (long workloads)."""
     print(f"[bench {time.strftime('%H:%M:%S')}] {message}", file=sys.stderr, flush=True)
@@ -409,6 +429,9 @@ def main():
                 "kernel_us_per_step_mean": round(mean_s * 1e6, 3),
                 "kernel_us_per_step_median": round(median_s * 1e6, 3),
                 "x_chunk": s.x_chunk, "resident": bool(s.resident)}
+    ceiling = copy_ceiling()
+    roofline["copy_ceiling_GBs"] = round(ceiling, 1)
+    roofline["frac_of_copy_ceiling"] = round(achieved / ceiling, 4)
     if s.resident:
         L = s.layout
         roofline["resident_layout"] = {"sub_box": [L.sx, L.sy, L.sz],
