@@ -264,6 +264,25 @@ int wfd_adjoint_gather_f64(float* q, const float* d, const float* hist, int hist
                            double* grad, const int* step, int nx, int ny, int nz, int x0,
                            int y0, int z0, int sx, int sy, int sz, int from_h,
                            void* stream);
+/* adjoint.py:46-162 prepare followed by gather in one pass, without d:
+ * from_h = 1 is prepare(q_src = qh, electric = 0) then gather into
+ * q_dst = qe (with the gradient); from_h = 0 is prepare(q_src = qe,
+ * electric = 1) then gather into q_dst = qh.  Reads psi_in, writes the
+ * updated psi to psi_out (distinct buffers) in the slots of axes with CPML
+ * at each cell; the other slots are never written, so callers alternating
+ * psi_in / psi_out must start both buffers equal there.  Results equal
+ * prepare + gather bit for bit. */
+int wfd_adjoint_reverse_half(float* q_dst, const float* q_src, const float* eps,
+                             const float* psi_in, float* psi_out, const float* coeff,
+                             int coeff_rows, const float* hist, int hist_steps, float* grad,
+                             const int* step, int nx, int ny, int nz, int x0, int y0, int z0,
+                             int sx, int sy, int sz, float courant, int from_h, void* stream);
+int wfd_adjoint_reverse_half_f64(float* q_dst, const float* q_src, const float* eps,
+                                 const float* psi_in, float* psi_out, const float* coeff,
+                                 int coeff_rows, const float* hist, int hist_steps, double* grad,
+                                 const int* step, int nx, int ny, int nz, int x0, int y0, int z0,
+                                 int sx, int sy, int sz, float courant, int from_h,
+                                 void* stream);
 /* adjoint.py:181-223 inject (transposed detector colocation, atomic scatter);
  * source (6, count, 2) float32 complex, weights (steps, 2). */
 int wfd_adjoint_inject(float* qe, float* qh, const float* source, const float* weights,

@@ -161,3 +161,46 @@ def test_checkpointed_adjoint_equals_full_history(pkg, gradient20):
     np.testing.assert_array_equal(adj.voxel_gradient(), voxel)
     np.testing.assert_array_equal(gradient2, gradient)
     assert adj.history.shape[0] == 64
+
+
+@pytest.mark.parametrize("from_h", [1, 0])
+def test_fused_reverse_half_equals_prepare_then_gather(pkg, gpu, from_h):
+    """wfd_adjoint_reverse_half (prepare + gather in one pass, psi to a second
+    buffer) equals prepare followed by gather bit for bit: q, psi and the FP64
+    gradient, with CPML slabs on every face and a design box (seed 5)."""
+    A = pkg.adjoint
+    rng = np.random.default_rng(5)
+    shape = (11, 9, 13)
+    n = int(np.prod(shape))
+    box = (2, 1, 3, 6, 5, 7)
+    steps = 4
+    q_src = _t(rng.normal(size=(3, n)))
+    q_dst = _t(rng.normal(size=(3, n)))
+    psi = rng.normal(size=(6, n)).astype(np.float32)
+    eps = _t(rng.uniform(0.1, 1, n))
+    coeff = np.zeros((3, max(shape), 3), np.float32)
+    coeff[:, :, 2] = 1
+    for axis, length in enumerate(shape):
+        for p in (0, 1, length - 2, length - 1):
+            coeff[axis, p] = rng.uniform(-0.5, 0.9, 3)
+    coeff = _t(coeff)
+    hist = _t(rng.normal(size=(steps, 3, box[3] * box[4] * box[5])))
+    step = torch.tensor([2], dtype=torch.int32, device="cuda")
+    grad0 = torch.from_numpy(rng.normal(size=box[3] * box[4] * box[5])).cuda()
+
+    # two-kernel reference
+    qa, psia, ga = q_dst.clone(), _t(psi), grad0.clone()
+    d = torch.empty((6, n), device="cuda")
+    A._prepare(q_src, eps, psia, coeff, d, *shape, 0.57, 1 - from_h)
+    A._gather(qa, d, hist, ga, step, *shape, *box, from_h)
+    # fused, psi_out starting as psi_in (slots without CPML are not written)
+    qb, gb = q_dst.clone(), grad0.clone()
+    psi_in, psi_out = _t(psi), _t(psi)
+    A._reverse_half(qb, q_src, eps, psi_in, psi_out, coeff, hist, gb, step, *shape, *box,
+                    0.57, from_h)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(qb.cpu().numpy(), qa.cpu().numpy())
+    np.testing.assert_array_equal(psi_out.cpu().numpy(), psia.cpu().numpy())
+    np.testing.assert_array_equal(psi_in.cpu().numpy(), psi)  # read only
+    np.testing.assert_array_equal(gb.cpu().numpy(), ga.cpu().numpy())
+    assert not np.array_equal(qb.cpu().numpy(), q_dst.cpu().numpy())

@@ -127,6 +127,8 @@ SIGNATURES = {
     "wfd_adjoint_prepare": [P, P, P, P, I, P, I, I, I, F, I, P],
     "wfd_adjoint_gather": [P, P, P, I, P, P] + [I] * 10 + [P],
     "wfd_adjoint_gather_f64": [P, P, P, I, P, P] + [I] * 10 + [P],
+    "wfd_adjoint_reverse_half": [P, P, P, P, P, P, I, P, I, P, P] + [I] * 9 + [F, I, P],
+    "wfd_adjoint_reverse_half_f64": [P, P, P, P, P, P, I, P, I, P, P] + [I] * 9 + [F, I, P],
     "wfd_adjoint_inject": [P, P, P, P, P] + [I] * 11 + [P],
     "wfd_write_design": [P, P] + [I] * 9 + [P],
     "wfd_driven_magnetic": [ctypes.POINTER(GuideDesc), P, P, P],

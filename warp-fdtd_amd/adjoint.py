@@ -46,6 +46,19 @@ def _gather(q, d, hist, grad, step, nx, ny, nz, x0, y0, z0, sx, sy, sz, from_h):
          stream())
 
 
+def _reverse_half(q_dst, q_src, eps, psi_in, psi_out, coeff, hist, grad, step, nx, ny, nz,
+                  x0, y0, z0, sx, sy, sz, courant, from_h):
+    """prepare + gather in one pass (wfd_adjoint_reverse_half), bitwise equal
+    to _prepare then _gather without the d streams."""
+    t = getattr(grad, "tensor", grad)
+    name = ("wfd_adjoint_reverse_half_f64" if t.dtype == torch.float64
+            else "wfd_adjoint_reverse_half")
+    call(name, ptr(q_dst), ptr(q_src), ptr(eps), ptr(psi_in), ptr(psi_out), ptr(coeff),
+         int(coeff.shape[1]), ptr(hist), int(hist.shape[0]), ptr(grad), ptr(step), int(nx),
+         int(ny), int(nz), int(x0), int(y0), int(z0), int(sx), int(sy), int(sz),
+         float(courant), int(from_h), stream())
+
+
 def _inject(qe, qh, source, weights, step, nx, ny, nz, x0, y0, z0, sy, sz, old_h_only,
             dim):
     call("wfd_adjoint_inject", ptr(qe), ptr(qh), ptr(source), ptr(weights), ptr(step),
@@ -118,9 +131,14 @@ class Adjoint(Solver):
         self.history = Array.empty((min(K, self.steps), 3, self.design_n), device=self.device)
         self.qe = Array.zeros((3, self.n), device=self.device)
         self.qh = Array.zeros((3, self.n), device=self.device)
+        # adjoint CPML psi: each reverse half-step reads one buffer and writes
+        # the other (neighbours still read the old values); _flip says which
+        # holds the state, and every captured chunk ends with it in the first
         self.psie = Array.zeros(self.psi_e.shape, device=self.device)
         self.psih = Array.zeros(self.psi_h.shape, device=self.device)
-        self.d = Array.empty((6, self.n), device=self.device)
+        self.psie2 = Array.zeros(self.psi_e.shape, device=self.device)
+        self.psih2 = Array.zeros(self.psi_h.shape, device=self.device)
+        self._flip = 0
         self.grad = Array.zeros(self.design_n, np.dtype(grad_dtype), self.device)
         self.reverse_step = Array.zeros(1, np.int32, self.device)
         self.adjoint_sources = []
@@ -301,22 +319,34 @@ class Adjoint(Solver):
         return m.weights
 
     def reverse_one(self):
-        """adjoint.py:388-414"""
+        """adjoint.py:388-414: inject, prepare H + gather qe, prepare E + gather
+        qh (each pair one pass), inject the H_old terms, retreat."""
         self.inject(0)
-        _prepare(self.qh, self.eps, self.psih, self.coeffs["H"], self.d, *self.shape,
-                 self.courant, 0)
-        _gather(self.qe, self.d, self.history, self.grad, self.reverse_step, *self.shape,
-                *self.box, 1)
-        _prepare(self.qe, self.eps, self.psie, self.coeffs["E"], self.d, *self.shape,
-                 self.courant, 1)
-        _gather(self.qh, self.d, self.history, self.grad, self.reverse_step, *self.shape,
-                *self.box, 0)
+        if self._flip:
+            e_in, e_out, h_in, h_out = self.psie2, self.psie, self.psih2, self.psih
+        else:
+            e_in, e_out, h_in, h_out = self.psie, self.psie2, self.psih, self.psih2
+        _reverse_half(self.qe, self.qh, self.eps, h_in, h_out, self.coeffs["H"], self.history,
+                      self.grad, self.reverse_step, *self.shape, *self.box, self.courant, 1)
+        _reverse_half(self.qh, self.qe, self.eps, e_in, e_out, self.coeffs["E"], self.history,
+                      self.grad, self.reverse_step, *self.shape, *self.box, self.courant, 0)
+        self._flip ^= 1
         self.inject(1)
         _retreat(self.reverse_step)
 
+    def _reverse_chunk(self, length):
+        """`length` reverse steps leaving the psi state in the first buffers."""
+        for _ in range(length):
+            self.reverse_one()
+        if self._flip:
+            self.psie.tensor.copy_(self.psie2.tensor)
+            self.psih.tensor.copy_(self.psih2.tensor)
+            self._flip = 0
+
     def reset_reverse(self):
-        for a in (self.qe, self.qh, self.psie, self.psih, self.grad):
+        for a in (self.qe, self.qh, self.psie, self.psih, self.psie2, self.psih2, self.grad):
             a.zero_()
+        self._flip = 0
         self.reverse_step.assign(np.array([self.steps - 1], np.int32))
         torch.cuda.synchronize(self.device)
 
@@ -331,7 +361,8 @@ class Adjoint(Solver):
         torch.cuda.synchronize(self.device)
         self.reset_reverse()
         for length in self._reverse_lengths():
-            self.reverse_graphs[length] = self._capture_fn(length, self.reverse_one)
+            self.reverse_graphs[length] = self._capture_fn(
+                1, lambda length=length: self._reverse_chunk(length))
         self.reset_reverse()
 
     def _reverse_lengths(self):

@@ -114,6 +114,103 @@ __global__ __launch_bounds__(kBlock) void k_gather(
   }
 }
 
+// Slots 2*axis and 2*axis + 1 of k_prepare's d at cell j (coordinate `pos`
+// along the axis), from q and psi before the update, with k_prepare's
+// operations; `p1`, `p2` return the updated psi (used when `active`).
+__device__ __forceinline__ void prepare_pair(const float* __restrict__ q,
+                                             const float* __restrict__ eps,
+                                             const float* __restrict__ psi,
+                                             const float* __restrict__ coeff, int rows, int n,
+                                             int j, int axis, int pos, float courant, int electric,
+                                             float& d1, float& d2, float& p1, float& p2,
+                                             bool& active) {
+  const float factor = electric ? courant * eps[j] : -courant;
+  const int a = (axis + 1) % 3, b = (axis + 2) % 3;
+  const float q1 = -(factor * q[a * n + j]);
+  const float q2 = factor * q[b * n + j];
+  const float* co = coeff + (axis * rows + pos) * 3;
+  d1 = co[2] * q1;
+  d2 = co[2] * q2;
+  active = co[0] != 0.f;
+  if (active) {
+    const float s1 = psi[(2 * axis) * n + j] + q1;
+    const float s2 = psi[(2 * axis + 1) * n + j] + q2;
+    d1 += co[0] * s1;
+    d2 += co[0] * s2;
+    p1 = co[1] * s1;
+    p2 = co[1] * s2;
+  }
+}
+
+// One k_prepare + k_gather pair in a single pass (adjoint.py:46-162): d is not
+// stored; each cell computes its own six slots and, for each axis, the two
+// slots of its neighbour along that axis (where the gather reads them), from
+// q_src and psi_in with k_prepare's operations, then sums them in k_gather's
+// order, so q_dst, psi and grad equal the two-kernel pass bit for bit.  The
+// updated psi goes to psi_out (a second buffer: neighbours read psi_in), in
+// the slots of the axes with CPML at the cell; other slots are never written
+// in either buffer.
+template <typename G>
+__global__ __launch_bounds__(kBlock) void k_reverse_half(
+    float* __restrict__ q_dst, const float* __restrict__ q_src, const float* __restrict__ eps,
+    const float* __restrict__ psi_in, float* __restrict__ psi_out,
+    const float* __restrict__ coeff, int rows, const float* __restrict__ hist, int hist_steps,
+    G* __restrict__ grad, const int* __restrict__ step, int nx, int ny, int nz, int x0, int y0,
+    int z0, int sx, int sy, int sz, float courant, int from_h) {
+  const int n = nx * ny * nz;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int x = i / (ny * nz);
+  const int y = (i / nz) % ny;
+  const int z = i % nz;
+  const int electric = from_h ? 0 : 1;
+  const int delta = from_h ? -1 : 1;
+  const int c[3] = {x, y, z};
+  const int dim[3] = {nx, ny, nz};
+  const int stride[3] = {ny * nz, nz, 1};
+  float own[6], nb[6];
+#pragma unroll
+  for (int axis = 0; axis < 3; ++axis) {
+    float p1 = 0.f, p2 = 0.f;
+    bool active;
+    prepare_pair(q_src, eps, psi_in, coeff, rows, n, i, axis, c[axis], courant, electric,
+                 own[2 * axis], own[2 * axis + 1], p1, p2, active);
+    if (active) {
+      psi_out[(2 * axis) * n + i] = p1;
+      psi_out[(2 * axis + 1) * n + i] = p2;
+    }
+    const int cn = c[axis] + delta;
+    if (cn >= 0 && cn < dim[axis]) {
+      float u1, u2;
+      prepare_pair(q_src, eps, psi_in, coeff, rows, n, i + delta * stride[axis], axis, cn,
+                   courant, electric, nb[2 * axis], nb[2 * axis + 1], u1, u2, active);
+    } else {
+      nb[2 * axis] = nb[2 * axis + 1] = 0.f;
+    }
+  }
+  float v0 = (nb[2] - own[2]) + (nb[5] - own[5]);
+  float v1 = (nb[1] - own[1]) + (nb[4] - own[4]);
+  float v2 = (nb[0] - own[0]) + (nb[3] - own[3]);
+  if (!from_h) {
+    v0 = -v0;
+    v1 = -v1;
+    v2 = -v2;
+  }
+  const float q0 = q_dst[i] + v0, q1 = q_dst[n + i] + v1, q2 = q_dst[2 * n + i] + v2;
+  q_dst[i] = q0;
+  q_dst[n + i] = q1;
+  q_dst[2 * n + i] = q2;
+  const int t = step[0];
+  if (from_h && t >= 0 && x >= x0 && x < x0 + sx && y >= y0 && y < y0 + sy && z >= z0 &&
+      z < z0 + sz) {
+    const int dn = sx * sy * sz;
+    const int j = ((x - x0) * sy + (y - y0)) * sz + (z - z0);
+    const int64_t base = (int64_t)(t % hist_steps) * 3 * dn + j;
+    const float contrib = hist[base] * q0 + hist[base + dn] * q1 + hist[base + 2 * dn] * q2;
+    grad[j] += (G)contrib;
+  }
+}
+
 // adjoint.py:165-223 — transposed colocation.  The reference scatters each
 // plane cell's six weighted values with atomics; here every target cell of
 // the stencil footprint gathers the same contributions in a fixed order, so
@@ -224,12 +321,58 @@ int launch_gather(float* q, const float* d, const float* hist, int hist_steps, G
   return WFD_OK;
 }
 
+template <typename G>
+int launch_reverse_half(float* q_dst, const float* q_src, const float* eps, const float* psi_in,
+                        float* psi_out, const float* coeff, int coeff_rows, const float* hist,
+                        int hist_steps, G* grad, const int* step, int nx, int ny, int nz, int x0,
+                        int y0, int z0, int sx, int sy, int sz, float courant, int from_h,
+                        void* stream) {
+  WFD_REQUIRE(q_dst && q_src && psi_in && psi_out && coeff && step && (from_h || eps),
+              "reverse_half: null pointer");
+  WFD_REQUIRE(psi_in != psi_out && q_dst != q_src, "reverse_half: buffers must be distinct");
+  if (int s = check_box(nx, ny, nz, x0, y0, z0, sx, sy, sz)) return s;
+  WFD_REQUIRE(coeff_rows >= nx && coeff_rows >= ny && coeff_rows >= nz,
+              "reverse_half: coeff_rows < max(shape)");
+  const bool has_box = sx * sy * sz > 0;
+  WFD_REQUIRE(!(from_h && has_box) || (hist && grad && hist_steps > 0),
+              "reverse_half: history and gradient required");
+  const int n = nx * ny * nz;
+  hipLaunchKernelGGL((k_reverse_half<G>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                     as_stream(stream), q_dst, q_src, eps, psi_in, psi_out, coeff, coeff_rows,
+                     hist, hist_steps > 0 ? hist_steps : 1, grad, step, nx, ny, nz, x0, y0, z0,
+                     sx, sy, sz, courant, from_h);
+  WFD_LAUNCH_CHECK("reverse_half");
+  return WFD_OK;
+}
+
 }  // namespace
 }  // namespace wfd
 
 using namespace wfd;
 
 extern "C" {
+
+int wfd_adjoint_reverse_half(float* q_dst, const float* q_src, const float* eps,
+                             const float* psi_in, float* psi_out, const float* coeff,
+                             int coeff_rows, const float* hist, int hist_steps, float* grad,
+                             const int* step, int nx, int ny, int nz, int x0, int y0, int z0,
+                             int sx, int sy, int sz, float courant, int from_h, void* stream) {
+  return launch_reverse_half<float>(q_dst, q_src, eps, psi_in, psi_out, coeff, coeff_rows, hist,
+                                    hist_steps, grad, step, nx, ny, nz, x0, y0, z0, sx, sy, sz,
+                                    courant, from_h, stream);
+}
+
+int wfd_adjoint_reverse_half_f64(float* q_dst, const float* q_src, const float* eps,
+                                 const float* psi_in, float* psi_out, const float* coeff,
+                                 int coeff_rows, const float* hist, int hist_steps, double* grad,
+                                 const int* step, int nx, int ny, int nz, int x0, int y0, int z0,
+                                 int sx, int sy, int sz, float courant, int from_h,
+                                 void* stream) {
+  return launch_reverse_half<double>(q_dst, q_src, eps, psi_in, psi_out, coeff, coeff_rows,
+                                     hist, hist_steps, grad, step, nx, ny, nz, x0, y0, z0, sx,
+                                     sy, sz, courant, from_h, stream);
+}
+
 
 int wfd_record_curl(const float* h, float* hist, int hist_steps, const int* step, int nx,
                     int ny, int nz, int x0, int y0, int z0, int sx, int sy, int sz,
