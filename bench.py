@@ -225,6 +225,54 @@ def bench_adjoint(args):
           fd_check=fd)
 
 
+def bench_slab(args):
+    """The 64^3 dielectric slab (10-cell CPML, y-polarised Gaussian plane pulse,
+    5-wavelength flux DFT planes, 600 steps): whole-job throughput and the
+    roofline of the step kernel over the timed run's launches."""
+    import torch
+
+    import __graft_entry__
+
+    pkg = __graft_entry__.package()
+    path = Path(tempfile.gettempdir()) / "wfdtd_bench_slab64"
+    if not (path / "manifest.json").exists():
+        pkg.cases.slab_case(path)
+    resident = {"auto": "auto", "on": True, "off": False}[args.resident]
+    s = pkg.solver.Solver(path, resident=resident, graph_steps=args.graph_steps)
+    pkg.detectors.attach(s)
+    s.compile()
+    steps = s.steps
+    s.reset()
+    s.run_steps(steps)
+    s.reset()
+    torch.cuda.synchronize()
+    launches = [] if s.resident else None
+    t0 = time.perf_counter()
+    s.run_steps(steps, synchronize=False, events=launches)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    s.check()
+    B, f_cpml = algorithmic_bytes_per_cell(s.shape, 10)
+    if s.resident:
+        per = np.array([a.elapsed_time(b) * 1e-3 / n for a, b, n in launches])
+        weights = np.array([n for _, _, n in launches], float)
+        mean_s = float((per * weights).sum() / weights.sum())
+    else:
+        mean_s = kernel_time_per_step(pkg, s, args.kernel_repeats)[1]
+    achieved = s.n * B / mean_s / 1e9
+    _line("Mcell-updates/s + % HBM roofline (64^3 slab, 600 steps)",
+          round(s.n * steps / elapsed / 1e6, 1), "Mcell-updates/s", steps, 1,
+          round(elapsed / steps * 1e3, 5),
+          {"workload": "64^3 vacuum + 16-cell eps 12.08 slab @ 50 nm, 10-cell CPML, Gaussian "
+                       "plane pulse, flux DFT planes at 5 wavelengths 1.45-1.65 um",
+           "cells": s.n, "resident": bool(s.resident),
+           "steps_per_launch": s.graph_steps if s.resident else None},
+          roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "kernel_us_per_step_mean": round(mean_s * 1e6, 3),
+                    "bytes_per_cell_update": round(B, 2), "f_cpml": round(f_cpml, 4)})
+
+
 def bench_batch(args):
     """32 uneven bend designs (120x120x64 .. 256x224x96, 3000 steps, 8-wavelength
     flux DFT planes + a modal objective): batched forward + per-design adjoint."""
@@ -352,7 +400,8 @@ def main():
     p.add_argument("--copy-mib", type=int, default=0,
                    help="only run 8 device-to-device copies of this many MiB (the reference "
                         "traffic for calibrating HBM counters) and exit")
-    p.add_argument("--workload", choices=("waveguide", "bend-adjoint", "batch", "surrogate"),
+    p.add_argument("--workload", choices=("waveguide", "slab", "bend-adjoint", "batch",
+                                           "surrogate"),
                    default="waveguide",
                    help="waveguide: the headline Yee-step metric; the others measure the "
                         "adjoint, batched-design and surrogate workloads")
@@ -364,7 +413,7 @@ def main():
                    help="surrogate: largest density change of a step")
     args = p.parse_args()
     if args.workload != "waveguide" and args.gpus == 1:
-        {"bend-adjoint": bench_adjoint, "batch": bench_batch,
+        {"slab": bench_slab, "bend-adjoint": bench_adjoint, "batch": bench_batch,
          "surrogate": bench_surrogate}[args.workload](args)
         return
     if args.copy_mib:

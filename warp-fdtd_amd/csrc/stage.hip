@@ -1,9 +1,12 @@
 // Deferred detector recording for gfx950.  The H half-step stages the raw E,
 // H and H_old of every detector footprint each step (yee.hip); once per graph
-// chunk one launch replays the steps in order, colocating each plane from the
-// staged values with exactly the arithmetic of detectors.py:26-62 and then
-// accumulating like detectors.py:99-106, so results equal per-step recording
-// bit for bit while the time loop carries no detector launches.
+// chunk one launch replays the steps, colocating each plane from the staged
+// values with exactly the arithmetic of detectors.py:26-62 and then
+// accumulating like detectors.py:99-106, so phasor and flux results equal
+// per-step recording bit for bit while the time loop carries no detector
+// launches.  FP64 DFT planes sum a chunk's steps (in a fixed order of step
+// segments) before adding them: deterministic, and within rounding of a
+// per-step running sum.
 #include "common.hpp"
 
 namespace wfd {
@@ -35,60 +38,62 @@ struct ReplayArgs {
   int count;
 };
 
-// Staged field f (0-2 E, 3-5 H, 6-8 H_old) at (x, y, z); zero outside the grid
-// (every in-grid cell the colocation touches lies in the footprint).
-struct Slot {
-  const float* base;
-  const Replay* r;
-  int nx, ny, nz;
-  __device__ __forceinline__ float at(int f, int x, int y, int z) const {
-    if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) return 0.f;
-    const int idx = ((x - r->fx0) * r->fsy + (y - r->fy0)) * r->fsz + (z - r->fz0);
-    return base[f * r->fcount + idx];
-  }
-  // detectors.py:10-23 for component c
-  __device__ __forceinline__ float ht(int c, int x, int y, int z) const {
-    return 0.5f * (at(3 + c, x, y, z) + at(6 + c, x, y, z));
-  }
-};
-
-// Component c (0-5: Ex Ey Ez Hx Hy Hz) of detectors.colocate at (x, y, z).
-__device__ __forceinline__ float colocated(const Slot& s, int c, int x, int y, int z,
-                                           int exact) {
-  if (!exact) return s.at(c, x, y, z);
-  switch (c) {
-    case 0:
-      return 0.25f * (((s.at(0, x, y, z) + s.at(0, x - 1, y, z)) + s.at(0, x, y, z + 1)) +
-                      s.at(0, x - 1, y, z + 1));
-    case 1:
-      return 0.25f * (((s.at(1, x, y, z) + s.at(1, x, y - 1, z)) + s.at(1, x, y, z + 1)) +
-                      s.at(1, x, y - 1, z + 1));
-    case 2:
-      return s.at(2, x, y, z);
-    case 3:
-      return 0.5f * (s.ht(0, x, y, z) + s.ht(0, x, y - 1, z));
-    case 4:
-      return 0.5f * (s.ht(1, x, y, z) + s.ht(1, x - 1, y, z));
-    default:
-      return 0.125f * (((((((s.ht(2, x, y, z) + s.ht(2, x - 1, y, z)) + s.ht(2, x, y - 1, z)) +
-                           s.ht(2, x - 1, y - 1, z)) +
-                          s.ht(2, x, y, z + 1)) +
-                         s.ht(2, x - 1, y, z + 1)) +
-                        s.ht(2, x, y - 1, z + 1)) +
-                       s.ht(2, x - 1, y - 1, z + 1));
-  }
-}
-
 __device__ __forceinline__ void plane_xyz(const Replay& r, int i, int& x, int& y, int& z) {
   x = r.x0 + i / (r.sy * r.sz);
   y = r.y0 + (i / r.sz) % r.sy;
   z = r.z0 + i % r.sz;
 }
 
+// The eight stencil points of a plane cell's colocation, as footprint
+// indices (-1 outside the grid): 0 (x,y,z), 1 (x-1,y,z), 2 (x,y-1,z),
+// 3 (x-1,y-1,z), 4-7 the same at z+1.  Computed once per thread, so a loop
+// over steps only adds the slot and field offsets.
+struct Points {
+  int idx[8];
+};
+
+__device__ __forceinline__ Points make_points(const Replay& r, int x, int y, int z, int nx,
+                                              int ny, int nz) {
+  Points k;
+#pragma unroll
+  for (int p = 0; p < 8; ++p) {
+    const int px = x - (p & 1), py = y - ((p >> 1) & 1), pz = z + (p >> 2);
+    const bool in = px >= 0 && py >= 0 && pz >= 0 && px < nx && py < ny && pz < nz;
+    k.idx[p] = in ? ((px - r.fx0) * r.fsy + (py - r.fy0)) * r.fsz + (pz - r.fz0) : -1;
+  }
+  return k;
+}
+
+// The colocation of component c (detectors.py:26-62, the same operations in
+// the same order) from a step's slot `base` through the point table.
+__device__ __forceinline__ float colocated_at(const Points& k, const float* __restrict__ base,
+                                              int fcount, int c, int exact) {
+  auto at = [&](int f, int p) { return k.idx[p] >= 0 ? base[f * fcount + k.idx[p]] : 0.f; };
+  auto ht = [&](int cc, int p) { return 0.5f * (at(3 + cc, p) + at(6 + cc, p)); };
+  if (!exact) return at(c, 0);
+  switch (c) {
+    case 0:
+      return 0.25f * (((at(0, 0) + at(0, 1)) + at(0, 4)) + at(0, 5));
+    case 1:
+      return 0.25f * (((at(1, 0) + at(1, 2)) + at(1, 4)) + at(1, 6));
+    case 2:
+      return at(2, 0);
+    case 3:
+      return 0.5f * (ht(0, 0) + ht(0, 2));
+    case 4:
+      return 0.5f * (ht(1, 0) + ht(1, 1));
+    default:
+      return 0.125f * (((((((ht(2, 0) + ht(2, 1)) + ht(2, 2)) + ht(2, 3)) + ht(2, 4)) +
+                         ht(2, 5)) +
+                        ht(2, 6)) +
+                       ht(2, 7));
+  }
+}
+
 // The staged values one colocated component reads, as byte offsets into a
 // step's slot (4 * (field * fcount + footprint index); past the slot's end
 // for points outside the grid, which a buffer load reads as 0), in the order
-// `colocated` consumes them, so a loop over steps only loads and evaluates
+// the colocation consumes them, so a loop over steps only loads and evaluates
 // the same expression: form 0 one value, 1 the four-point E average, 2 the
 // two-point H average of (H + H_old) / 2, 3 the eight-point one.
 struct Terms {
@@ -137,9 +142,10 @@ __device__ __forceinline__ Terms make_terms(const Replay& r, int c, int x, int y
   return k;
 }
 
-// colocated() of one step through precomputed terms: the same operations in
-// the same order.  `ring` is a buffer over the whole stage ring and `so` the
-// step's slot offset in bytes (an outside term stays past the ring's end).
+// One component's colocation of one step through precomputed terms: the
+// same operations in the same order.  `ring` is a buffer over the whole stage
+// ring and `so` the step's slot offset in bytes (an outside term stays past
+// the ring's end).
 __device__ __forceinline__ float eval_terms(const Terms& k, Rsrc ring, unsigned so) {
   auto v = [&](int j) { return ld(ring, (int)((unsigned)k.off[j] + so), 0); };
   switch (k.form) {
@@ -166,11 +172,6 @@ __global__ __launch_bounds__(kBlock) void k_replay(ReplayArgs args, int slots,
   while (k + 1 < args.count && b >= args.begin[k + 1]) ++k;
   const Replay& r = args.r[k];
   const int last = step[0];
-  Slot s;
-  s.r = &r;
-  s.nx = nx;
-  s.ny = ny;
-  s.nz = nz;
   const int local = b - args.begin[k];
   if (r.kind == 1) {  // flux: one block, plane reduced per step in a fixed order
     __shared__ float part[kBlock];
@@ -179,15 +180,16 @@ __global__ __launch_bounds__(kBlock) void k_replay(ReplayArgs args, int slots,
       if (t < 0 || t >= r.steps) continue;
       const int index = r.indices[t];
       if (index < 0) continue;
-      s.base = r.stage + (size_t)(t % slots) * 9 * r.fcount;
+      const float* base = r.stage + (size_t)(t % slots) * 9 * r.fcount;
       float acc = 0.f;
       for (int i = threadIdx.x; i < r.count; i += kBlock) {
         int x, y, z;
         plane_xyz(r, i, x, y, z);
+        const Points pts = make_points(r, x, y, z, nx, ny, nz);
         float e[3], h[3];
         for (int c = 0; c < 3; ++c) {
-          e[c] = colocated(s, c, x, y, z, r.exact);
-          h[c] = colocated(s, 3 + c, x, y, z, r.exact);
+          e[c] = colocated_at(pts, base, r.fcount, c, r.exact);
+          h[c] = colocated_at(pts, base, r.fcount, 3 + c, r.exact);
         }
         float cr;
         if (r.axis == 0) cr = e[1] * h[2] - e[2] * h[1];
@@ -211,6 +213,7 @@ __global__ __launch_bounds__(kBlock) void k_replay(ReplayArgs args, int slots,
     if (i >= r.count) return;
     int x, y, z;
     plane_xyz(r, i, x, y, z);
+    const Points pts = make_points(r, x, y, z, nx, ny, nz);
     float2 acc[6];
     float2* ph = reinterpret_cast<float2*>(r.phasor);
     for (int c = 0; c < 6; ++c) acc[c] = ph[c * r.count + i];
@@ -218,10 +221,10 @@ __global__ __launch_bounds__(kBlock) void k_replay(ReplayArgs args, int slots,
       const int t = last - m + 1 + j;
       if (t < 0 || t >= r.steps) continue;
       if (r.indices[t] < 0) continue;
-      s.base = r.stage + (size_t)(t % slots) * 9 * r.fcount;
+      const float* base = r.stage + (size_t)(t % slots) * 9 * r.fcount;
       const float wr = r.weights[2 * t], wi = r.weights[2 * t + 1];
       for (int c = 0; c < 6; ++c) {
-        const float v = colocated(s, c, x, y, z, r.exact);
+        const float v = colocated_at(pts, base, r.fcount, c, r.exact);
         acc[c].x += wr * v;
         acc[c].y += wi * v;
       }
