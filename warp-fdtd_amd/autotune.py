@@ -107,7 +107,8 @@ def measure_resident(solver, layout, steps=64, repeats=5):
 
 def candidate_layouts(solver, count=5):
     """Resident sub-box candidates: for each (sy, sz) the library accepts, its
-    layout with the fewest x planes; the `count` best by (planes, surface)."""
+    layout with the fewest x planes; the `count` best by (planes, surface),
+    and the best two with 2 and 4 times their x planes."""
     nx, ny, nz = solver.shape
     seen, out = set(), []
     zs = sorted({v for v in range(4, nz + 1, 4)} | {nz})
@@ -126,7 +127,20 @@ def candidate_layouts(solver, count=5):
             seen.add(key)
             out.append(L)
     out.sort(key=lambda L: (L.sx, L.sx * L.sy + L.sx * L.sz + L.sy * L.sz, L.blocks))
-    return out[:count]
+    out = out[:count]
+    # deeper sub-boxes (fewer workgroups, more x planes per column): on small
+    # grids they hide the x-face exchange behind the interior planes
+    for L in list(out[:2]):
+        for factor in (2, 4):
+            try:
+                M = solver.resident_layout(L.sx * factor, L.sy, L.sz)
+            except ValueError:
+                continue
+            key = (M.sx, M.sy, M.sz)
+            if key not in seen:
+                seen.add(key)
+                out.append(M)
+    return out
 
 
 def tune_resident(solver, use_cache=True, require=False):
@@ -134,7 +148,8 @@ def tune_resident(solver, use_cache=True, require=False):
     grid (or in any case with ``require``), else None.  Candidates: the
     library's own choice and the best few (sy, sz) factorisations
     (candidate_layouts), each timed on the grid."""
-    key = _key(solver) + "|resident-layout"
+    # (keyed apart from results timed over shallower candidate sets)
+    key = _key(solver) + "|resident-layout-deep"
     if use_cache:
         hit = _memory.get(key)
         if hit is None:
