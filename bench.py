@@ -309,7 +309,9 @@ def bench_batch(args):
            "designs": 32, "cells": bt.cells, "forward_s": round(fwd, 3),
            "adjoint_s": round(bwd, 3), "designs_per_s_forward_adjoint": round(32 / (fwd + bwd), 4),
            "generation_s": round(gen, 1), "T_mean": float(np.mean(T)),
-           "x_chunks": sorted({s.x_chunk for s in bt.batch.solvers})})
+           "x_chunks": sorted({s.x_chunk for s in bt.batch.solvers}),
+           "batched_step_us_by_x_chunk": {k: round(v, 1)
+                                          for k, v in bt.batch.x_chunk_us.items()}})
 
 
 def bench_surrogate(args):

@@ -5,7 +5,9 @@ steps) and advances all of them with ``wfd_batch_step``: one launch per
 half-step and x-chunk class covers every design's blocks, each design keeps
 its own fields, CPML, source, counters and staged detectors, and its results
 are bitwise those of stepping it alone.  ``compile`` tunes every design's x
-chunk (autotune.py, cached by shape), writes the device argument tables and
+chunk (autotune.py, cached by shape), then times the batched step with those
+and with one common chunk for all designs and keeps the fastest, writes the
+device argument tables and
 captures one HIP graph per chunk of ``graph_steps`` steps that steps every
 design and replays every design's staged detectors; ``run`` replays them.
 """
@@ -85,6 +87,32 @@ class Batch:
         return graph
 
     # ------------------------------------------------------------ driver
+    def tune_x_chunk(self, candidates=(1, 2, 4, 8), steps=6):
+        """Time the whole batched step (quiet tables, HIP events) with every
+        design at each x chunk and keep the fastest common chunk if it beats
+        the designs' own tuned chunks.  Returns {label: microseconds}."""
+        own = [s.x_chunk for s in self.solvers]
+        options = [("own", own)] + [(str(c), [c] * len(self.solvers)) for c in candidates]
+        timings = {}
+        for label, chunks in options:
+            for s, c in zip(self.solvers, chunks):
+                s.x_chunk = c
+            plans, ws = self._table(False)
+            call("wfd_batch_step", plans, len(self.solvers), ptr(ws), stream())
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(steps):
+                call("wfd_batch_step", plans, len(self.solvers), ptr(ws), stream())
+            b.record()
+            torch.cuda.synchronize(self.device)
+            timings[label] = a.elapsed_time(b) * 1e3 / steps
+        best = min(options, key=lambda o: timings[o[0]])
+        for s, c in zip(self.solvers, best[1]):
+            s.x_chunk = c
+        for s in self.solvers:
+            s.reset()
+        return timings
+
     def compile(self):
         start = time.perf_counter()
         for s in self.solvers:
@@ -93,6 +121,7 @@ class Batch:
             torch.cuda.synchronize(self.device)
             s.tune()
             s.reset()
+        self.x_chunk_us = self.tune_x_chunk()
         self._destroy_graphs()
         self._tables = {True: self._table(True), False: self._table(False)}
         for length in {self.graph_steps, self.steps % self.graph_steps} - {0}:

@@ -455,10 +455,19 @@ __global__ __launch_bounds__(kBlock) void k_yee_h(YeeArgs a) {
 // of `count` plans of one x-chunk class; first[d] is the first block of plan
 // d (first[count] = the grid size).  The per-cell arithmetic is the single
 // plan's, so every plan's result is bitwise its own wfd_yee_step's.
+// The design of block `bid`: the largest d with first[d] <= bid (first is
+// nondecreasing, first[0] = 0), by binary search: at most log2(count)
+// dependent loads before the block starts.
 __device__ __forceinline__ int batch_member(const int* __restrict__ first, int count, int bid) {
-  int d = 0;
-  while (d + 1 < count && first[d + 1] <= bid) ++d;
-  return d;
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= bid)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
 }
 
 template <int XC>
