@@ -163,6 +163,18 @@ def test_checkpointed_adjoint_equals_full_history(pkg, gradient20):
     assert adj.history.shape[0] == 64
 
 
+def test_odd_checkpoint_chunks_equal_full_history(pkg, gradient20):
+    """Checkpoints every 321 steps (chunks of 321 and a 109-step tail: every
+    reverse chunk odd, so each ends by copying the second psi buffers back)
+    give the full-history gradient bit for bit."""
+    p0, loss, gradient, voxel, meas = gradient20
+    adj = pkg.adjoint.Adjoint(BUNDLES / "gradient20_highest_exact", checkpoint_every=321)
+    pkg.detectors.attach(adj)
+    loss2, gradient2 = adj.differentiate(p0)
+    assert loss2 == loss
+    np.testing.assert_array_equal(gradient2, gradient)
+
+
 @pytest.mark.parametrize("from_h", [1, 0])
 def test_fused_reverse_half_equals_prepare_then_gather(pkg, gpu, from_h):
     """wfd_adjoint_reverse_half (prepare + gather in one pass, psi to a second
