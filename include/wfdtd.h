@@ -289,6 +289,20 @@ int wfd_adjoint_inject(float* qe, float* qh, const float* source, const float* w
                        const int* step, int nx, int ny, int nz, int x0, int y0, int z0,
                        int sy, int sz, int count, int old_h_only, int steps,
                        void* stream);
+/* The inject of several planes (adjoint.py:181-223, the loop over the
+ * objective's monitors) in one launch when their target boxes are disjoint,
+ * else one launch per plane in order; the results equal one
+ * wfd_adjoint_inject per plane, in order. */
+#define WFD_MAX_INJECT 8
+typedef struct wfd_inject_source {
+  const float* source;   /* (6, count, 2) float32 complex */
+  const float* weights;  /* (steps, 2) */
+  int x0, y0, z0, sy, sz, count, steps;
+} wfd_inject_source;
+
+int wfd_adjoint_inject_many(const wfd_inject_source* sources, int count, float* qe,
+                            float* qh, const int* step, int nx, int ny, int nz, int old_h_only,
+                            void* stream);
 /* adjoint.py:472-490 write_design: plane (sx, sy) into every z-layer of the box. */
 int wfd_write_design(const float* plane, float* eps, int nx, int ny, int nz, int x0,
                      int y0, int z0, int sx, int sy, int sz, void* stream);

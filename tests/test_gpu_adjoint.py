@@ -216,3 +216,48 @@ def test_fused_reverse_half_equals_prepare_then_gather(pkg, gpu, from_h):
     np.testing.assert_array_equal(psi_in.cpu().numpy(), psi)  # read only
     np.testing.assert_array_equal(gb.cpu().numpy(), ga.cpu().numpy())
     assert not np.array_equal(qb.cpu().numpy(), q_dst.cpu().numpy())
+
+
+@pytest.mark.parametrize("overlap", [False, True])
+def test_inject_many_equals_one_launch_per_plane(pkg, gpu, overlap):
+    """wfd_adjoint_inject_many (every plane in one launch; a target in several
+    planes' footprints is owned by the first plane's thread, which adds them
+    in order) equals wfd_adjoint_inject per plane in order, bit for bit, for
+    E+H and H_old-only injections (seed 7)."""
+    A = pkg.adjoint
+    lib = pkg._lib
+    rng = np.random.default_rng(7)
+    shape = (20, 12, 14)
+    n = int(np.prod(shape))
+    steps = 5
+    # (x0, y0, z0, sx, sy, sz): two x-normal planes and a third far away, or
+    # a y-normal plane crossing both footprints (as the two mode planes of a
+    # bend do) followed by an x-normal plane next to the first
+    planes = [(3, 2, 1, 1, 8, 10), (12, 1, 2, 1, 9, 9), (17, 3, 0, 1, 7, 11)]
+    if overlap:
+        planes[2:] = [(2, 5, 0, 12, 1, 11), (4, 3, 0, 1, 7, 11)]
+    srcs = []
+    for x0, y0, z0, sx, sy, sz in planes:
+        count = sx * sy * sz
+        source = _t(rng.normal(size=(6, count, 2)))
+        weights = rng.normal(size=(steps, 2)).astype(np.float32)
+        weights[1] = 0  # a step without injection
+        srcs.append((source, _t(weights), (x0, y0, z0), (sy, sz), count))
+    table = (lib.InjectSource * len(srcs))(*[
+        lib.InjectSource(s.data_ptr(), w.data_ptr(), *lo, *syz, c, steps)
+        for s, w, lo, syz, c in srcs])
+    qe0 = _t(rng.normal(size=(3, n)))
+    qh0 = _t(rng.normal(size=(3, n)))
+    for t in (0, 1, 4):
+        step = torch.tensor([t], dtype=torch.int32, device="cuda")
+        for old in (0, 1):
+            qa, qha = qe0.clone(), qh0.clone()
+            for s, w, lo, syz, c in srcs:
+                A._inject(qa, qha, s, w, step, *shape, *lo, *syz, old, dim=c)
+            qb, qhb = qe0.clone(), qh0.clone()
+            A._inject_many(table, qb, qhb, step, *shape, old)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(qb.cpu().numpy(), qa.cpu().numpy())
+            np.testing.assert_array_equal(qhb.cpu().numpy(), qha.cpu().numpy())
+            changed = not np.array_equal(qhb.cpu().numpy(), qh0.cpu().numpy())
+            assert changed == (t != 1)

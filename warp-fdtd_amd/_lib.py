@@ -48,6 +48,16 @@ class YeePlan(ctypes.Structure):
     ]
 
 
+class InjectSource(ctypes.Structure):
+    """wfd_inject_source (include/wfdtd.h)."""
+
+    _fields_ = [("source", P), ("weights", P), ("x0", I), ("y0", I), ("z0", I), ("sy", I),
+                ("sz", I), ("count", I), ("steps", I)]
+
+
+WFD_MAX_INJECT = 8
+
+
 class ResidentLayout(ctypes.Structure):
     """wfd_resident_layout (include/wfdtd.h)."""
 
@@ -130,6 +140,7 @@ SIGNATURES = {
     "wfd_adjoint_reverse_half": [P, P, P, P, P, P, I, P, I, P, P] + [I] * 9 + [F, I, P],
     "wfd_adjoint_reverse_half_f64": [P, P, P, P, P, P, I, P, I, P, P] + [I] * 9 + [F, I, P],
     "wfd_adjoint_inject": [P, P, P, P, P] + [I] * 11 + [P],
+    "wfd_adjoint_inject_many": [P, I, P, P, P, I, I, I, I, P],
     "wfd_write_design": [P, P] + [I] * 9 + [P],
     "wfd_driven_magnetic": [ctypes.POINTER(GuideDesc), P, P, P],
     "wfd_driven_electric": [ctypes.POINTER(GuideDesc), P, P, P, P, P],

@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import kernels as k
-from ._lib import call, ptr, stream
+from ._lib import WFD_MAX_INJECT, InjectSource, call, ptr, stream
 from .array import Array
 from .kernels import Kernel
 from .solver import Solver
@@ -64,6 +64,13 @@ def _inject(qe, qh, source, weights, step, nx, ny, nz, x0, y0, z0, sy, sz, old_h
     call("wfd_adjoint_inject", ptr(qe), ptr(qh), ptr(source), ptr(weights), ptr(step),
          int(nx), int(ny), int(nz), int(x0), int(y0), int(z0), int(sy), int(sz), int(dim),
          int(old_h_only), int(len(weights)), stream())
+
+
+def _inject_many(sources, qe, qh, step, nx, ny, nz, old_h_only):
+    """Every adjoint source plane of one inject in one launch: sources is a
+    ctypes array of InjectSource."""
+    call("wfd_adjoint_inject_many", ctypes.cast(sources, ctypes.c_void_p), len(sources),
+         ptr(qe), ptr(qh), ptr(step), int(nx), int(ny), int(nz), int(old_h_only), stream())
 
 
 def _retreat(step):
@@ -307,11 +314,32 @@ class Adjoint(Solver):
             self.adjoint_sources.append((m, v))
 
     def inject(self, old):
+        """adjoint.py:181-223: add every adjoint source plane's terms to qe, qh
+        (one launch for up to WFD_MAX_INJECT planes)."""
+        if not self.adjoint_sources:
+            return
+        if len(self.adjoint_sources) > WFD_MAX_INJECT:
+            for m, v in self.adjoint_sources:
+                lo = [a for a, b in m.obj["grid_slice"]]
+                _inject(self.qe, self.qh, v, self._inject_weights_of(m), self.reverse_step,
+                        *self.shape, *lo, *m.shape[1:], old, dim=m.n)
+            return
+        _inject_many(self._inject_table(), self.qe, self.qh, self.reverse_step, *self.shape,
+                     old)
+
+    def _inject_table(self):
+        rows = []
         for m, v in self.adjoint_sources:
             lo = [a for a, b in m.obj["grid_slice"]]
             weights = self._inject_weights_of(m)
-            _inject(self.qe, self.qh, v, weights, self.reverse_step, *self.shape, *lo,
-                    *m.shape[1:], old, dim=m.n)
+            rows.append((ptr(v).value, ptr(weights).value, *lo, *m.shape[1:], m.n,
+                         len(weights)))
+        rows = tuple(rows)
+        if getattr(self, "_inject_rows", None) != rows:
+            self._inject_rows = rows
+            self._inject_sources = (InjectSource * len(rows))(*[InjectSource(*r)
+                                                                 for r in rows])
+        return self._inject_sources
 
     def _inject_weights_of(self, m):
         if hasattr(m, "nf"):

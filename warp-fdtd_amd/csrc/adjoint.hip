@@ -231,36 +231,39 @@ __device__ __forceinline__ float plane_value(const InjectPlane& p, int c, int x,
   return s.x * p.w0 - s.y * p.w1;
 }
 
-__global__ __launch_bounds__(kBlock) void k_inject(
-    float* __restrict__ qe, float* __restrict__ qh, const float* __restrict__ source,
-    const float* __restrict__ weights, const int* __restrict__ step, int nx, int ny, int nz,
-    int x0, int y0, int z0, int sy, int sz, int count, int old_h_only, int steps) {
-  // targets: x in [x0-1, x0+sx), y in [y0-1, y0+sy), z in [z0, z0+sz+1)
-  const int sx = count / (sy * sz);
-  const int tx0 = x0 - 1, ty0 = y0 - 1;
-  const int tsx = sx + 1, tsy = sy + 1, tsz = sz + 1;
-  const int k = blockIdx.x * kBlock + threadIdx.x;
-  if (k >= tsx * tsy * tsz) return;
-  const int t = step[0];
-  if (t < 0 || t >= steps) return;
+// One plane's adjoint source for the reverse step (a wfd_inject_source).
+struct InjectSource {
+  const float* source;
+  const float* weights;
+  int x0, y0, z0, sy, sz, count, steps;
+};
+
+// The plane's stencil footprint (its target box): x in [x0-1, x0+sx),
+// y in [y0-1, y0+sy), z in [z0, z0+sz+1).
+__device__ __forceinline__ bool in_targets(const InjectSource& g, int x, int y, int z) {
+  const int sx = g.count / (g.sy * g.sz);
+  return x >= g.x0 - 1 && x < g.x0 + sx && y >= g.y0 - 1 && y < g.y0 + g.sy && z >= g.z0 &&
+         z <= g.z0 + g.sz;
+}
+
+// Adds plane g's contributions at step t to target (x, y, z), in a fixed
+// order: e[0..2] (unless old_h_only) and h[0..2].  A step with zero weights
+// (or outside [0, steps)) adds nothing.
+__device__ __forceinline__ void add_plane(const InjectSource& g, int t, int x, int y, int z,
+                                          int old_h_only, float* e, float* h) {
+  if (t < 0 || t >= g.steps) return;
   InjectPlane p;
-  p.w0 = weights[2 * t];
-  p.w1 = weights[2 * t + 1];
+  p.w0 = g.weights[2 * t];
+  p.w1 = g.weights[2 * t + 1];
   if (!(fabsf(p.w0) + fabsf(p.w1) > 0.f)) return;
-  const int x = tx0 + k / (tsy * tsz);
-  const int y = ty0 + (k / tsz) % tsy;
-  const int z = z0 + k % tsz;
-  if (x < 0 || y < 0 || x >= nx || y >= ny || z >= nz) return;
-  p.source = reinterpret_cast<const float2*>(source);
-  p.x0 = x0;
-  p.y0 = y0;
-  p.z0 = z0;
-  p.sx = sx;
-  p.sy = sy;
-  p.sz = sz;
-  p.count = count;
-  const int n = nx * ny * nz;
-  const int i = (x * ny + y) * nz + z;
+  p.source = reinterpret_cast<const float2*>(g.source);
+  p.x0 = g.x0;
+  p.y0 = g.y0;
+  p.z0 = g.z0;
+  p.sx = g.count / (g.sy * g.sz);
+  p.sy = g.sy;
+  p.sz = g.sz;
+  p.count = g.count;
   // scatter target (x-dx, y, z+dz) of E_x <- plane cell (x+dx, y, z-dz), etc.
   if (!old_h_only) {
     float a0 = 0.f, a1 = 0.f;
@@ -268,9 +271,9 @@ __global__ __launch_bounds__(kBlock) void k_inject(
       for (int dx = 0; dx < 2; ++dx) a0 += 0.25f * plane_value(p, 0, x + dx, y, z - dz);
       for (int dy = 0; dy < 2; ++dy) a1 += 0.25f * plane_value(p, 1, x, y + dy, z - dz);
     }
-    qe[i] += a0;
-    qe[n + i] += a1;
-    qe[2 * n + i] += plane_value(p, 2, x, y, z);
+    e[0] += a0;
+    e[1] += a1;
+    e[2] += plane_value(p, 2, x, y, z);
   }
   float b0 = 0.f, b1 = 0.f, b2 = 0.f;
   for (int dy = 0; dy < 2; ++dy) b0 += 0.25f * plane_value(p, 3, x, y + dy, z);
@@ -278,9 +281,52 @@ __global__ __launch_bounds__(kBlock) void k_inject(
   for (int dx = 0; dx < 2; ++dx)
     for (int dy = 0; dy < 2; ++dy)
       for (int dz = 0; dz < 2; ++dz) b2 += 0.0625f * plane_value(p, 5, x + dx, y + dy, z - dz);
-  qh[i] += b0;
-  qh[n + i] += b1;
-  qh[2 * n + i] += b2;
+  h[0] += b0;
+  h[1] += b1;
+  h[2] += b2;
+}
+
+// Planes g[0..count) in one launch.  Blocks are split per plane (begin: the
+// first block of each plane) over its target box; a target that also lies in
+// an earlier plane's box is left to that plane's thread, and the thread that
+// owns a target adds every plane containing it in plane order.  Each cell is
+// written by one thread, with the roundings of one launch per plane in order.
+struct InjectMany {
+  InjectSource g[WFD_MAX_INJECT];
+  int begin[WFD_MAX_INJECT + 1];
+  int count;
+};
+
+__global__ __launch_bounds__(kBlock) void k_inject_many(InjectMany a, float* __restrict__ qe,
+                                                        float* __restrict__ qh,
+                                                        const int* __restrict__ step, int nx,
+                                                        int ny, int nz, int old_h_only) {
+  const int b = blockIdx.x;
+  int k = 0;
+  while (k + 1 < a.count && b >= a.begin[k + 1]) ++k;
+  const InjectSource& g = a.g[k];
+  const int sx = g.count / (g.sy * g.sz);
+  const int tsy = g.sy + 1, tsz = g.sz + 1;
+  const int r = (b - a.begin[k]) * kBlock + threadIdx.x;
+  if (r >= (sx + 1) * tsy * tsz) return;
+  const int x = g.x0 - 1 + r / (tsy * tsz);
+  const int y = g.y0 - 1 + (r / tsz) % tsy;
+  const int z = g.z0 + r % tsz;
+  if (x < 0 || y < 0 || x >= nx || y >= ny || z >= nz) return;
+  for (int j = 0; j < k; ++j)
+    if (in_targets(a.g[j], x, y, z)) return;
+  const int t = step[0];
+  const int n = nx * ny * nz;
+  const int i = (x * ny + y) * nz + z;
+  float e[3] = {0.f, 0.f, 0.f}, h[3];
+  if (!old_h_only)
+    for (int c = 0; c < 3; ++c) e[c] = qe[c * n + i];
+  for (int c = 0; c < 3; ++c) h[c] = qh[c * n + i];
+  for (int l = k; l < a.count; ++l)
+    if (l == k || in_targets(a.g[l], x, y, z)) add_plane(a.g[l], t, x, y, z, old_h_only, e, h);
+  if (!old_h_only)
+    for (int c = 0; c < 3; ++c) qe[c * n + i] = e[c];
+  for (int c = 0; c < 3; ++c) qh[c * n + i] = h[c];
 }
 
 // adjoint.py:472-490
@@ -422,12 +468,35 @@ int wfd_adjoint_inject(float* qe, float* qh, const float* source, const float* w
                        int sy, int sz, int count, int old_h_only, int steps, void* stream) {
   WFD_REQUIRE(qe && qh && source && weights && step, "inject: null pointer");
   WFD_REQUIRE(count > 0 && sy > 0 && sz > 0 && count % (sy * sz) == 0, "inject: bad plane");
-  if (int s = check_box(nx, ny, nz, x0, y0, z0, count / (sy * sz), sy, sz)) return s;
-  const int targets = (count / (sy * sz) + 1) * (sy + 1) * (sz + 1);
-  hipLaunchKernelGGL(k_inject, dim3((targets + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                     as_stream(stream), qe, qh, source, weights, step, nx, ny, nz, x0, y0, z0,
-                     sy, sz, count, old_h_only, steps);
-  WFD_LAUNCH_CHECK("inject");
+  const wfd_inject_source one{source, weights, x0, y0, z0, sy, sz, count, steps};
+  return wfd_adjoint_inject_many(&one, 1, qe, qh, step, nx, ny, nz, old_h_only, stream);
+}
+
+int wfd_adjoint_inject_many(const wfd_inject_source* sources, int count, float* qe,
+                            float* qh, const int* step, int nx, int ny, int nz, int old_h_only,
+                            void* stream) {
+  WFD_REQUIRE(sources && qe && qh && step, "inject_many: null pointer");
+  WFD_REQUIRE(count >= 0 && count <= WFD_MAX_INJECT, "inject_many: 0..%d planes",
+              WFD_MAX_INJECT);
+  InjectMany a;
+  a.count = count;
+  int blocks = 0;
+  for (int k = 0; k < count; ++k) {
+    const wfd_inject_source& d = sources[k];
+    WFD_REQUIRE(d.source && d.weights && d.steps > 0, "inject_many: plane %d arrays", k);
+    WFD_REQUIRE(d.count > 0 && d.sy > 0 && d.sz > 0 && d.count % (d.sy * d.sz) == 0,
+                "inject_many: plane %d shape", k);
+    const int sx = d.count / (d.sy * d.sz);
+    if (int s = check_box(nx, ny, nz, d.x0, d.y0, d.z0, sx, d.sy, d.sz)) return s;
+    a.g[k] = InjectSource{d.source, d.weights, d.x0, d.y0, d.z0, d.sy, d.sz, d.count, d.steps};
+    a.begin[k] = blocks;
+    blocks += ((sx + 1) * (d.sy + 1) * (d.sz + 1) + kBlock - 1) / kBlock;
+  }
+  a.begin[count] = blocks;
+  if (count == 0) return WFD_OK;
+  hipLaunchKernelGGL(k_inject_many, dim3(blocks), dim3(kBlock), 0, as_stream(stream), a, qe,
+                     qh, step, nx, ny, nz, old_h_only);
+  WFD_LAUNCH_CHECK("inject_many");
   return WFD_OK;
 }
 
