@@ -283,16 +283,19 @@ int wfd_adjoint_reverse_half_f64(float* q_dst, const float* q_src, const float* 
                                  const int* step, int nx, int ny, int nz, int x0, int y0, int z0,
                                  int sx, int sy, int sz, float courant, int from_h,
                                  void* stream);
-/* adjoint.py:181-223 inject (transposed detector colocation, atomic scatter);
- * source (6, count, 2) float32 complex, weights (steps, 2). */
+/* adjoint.py:181-223 inject (the transposed detector colocation; the
+ * reference scatters with atomics, here each target cell gathers its terms
+ * in a fixed order); source (6, count, 2) float32 complex, weights
+ * (steps, 2). */
 int wfd_adjoint_inject(float* qe, float* qh, const float* source, const float* weights,
                        const int* step, int nx, int ny, int nz, int x0, int y0, int z0,
                        int sy, int sz, int count, int old_h_only, int steps,
                        void* stream);
 /* The inject of several planes (adjoint.py:181-223, the loop over the
- * objective's monitors) in one launch when their target boxes are disjoint,
- * else one launch per plane in order; the results equal one
- * wfd_adjoint_inject per plane, in order. */
+ * objective's monitors) in one launch, count <= WFD_MAX_INJECT.  A cell in
+ * several planes' stencil footprints is updated by one thread, which adds
+ * the planes in order, so the results equal one wfd_adjoint_inject per
+ * plane, in order, bit for bit. */
 #define WFD_MAX_INJECT 8
 typedef struct wfd_inject_source {
   const float* source;   /* (6, count, 2) float32 complex */
