@@ -290,6 +290,18 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   };
 
+  // the plane source's profile at the thread's columns, loaded once per
+  // launch (a load inside the march would stall the source slab's
+  // workgroups, and their neighbours wait on them)
+  float src_ey = 0.f, src_ez = 0.f, src_hy = 0.f, src_hz = 0.f;
+  if (active && a.source_x >= X0 && a.source_x < X0 + vx) {
+    const int oe = (Y0 + eyl) * nz + Z0 + ezl, oh = (Y0 + hyl) * nz + Z0 + hzl;
+    src_ey = a.src_h[2 * P + oe];
+    src_ez = a.src_h[P + oe];
+    src_hy = a.src_e[2 * P + oh];
+    src_hz = a.src_e[P + oh];
+  }
+
   for (int k = 0; k < nsteps; ++k) {
     const int t = t0 + k;
     // ------------------------------------------------------------ E half
@@ -392,8 +404,8 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
         float ey = E[cells + c] + (S * (dzx - dxz)) * ie;
         float ez = E[2 * cells + c] + (S * (dxy - dyx)) * ie;
         if (src_here && x == a.source_x) {
-          ey += ((a.src_h[2 * P + o] * wave_y) * S) * ie;
-          ez -= ((a.src_h[P + o] * wave_z) * S) * ie;
+          ey += ((src_ey * wave_y) * S) * ie;
+          ez -= ((src_ez * wave_z) * S) * ie;
         }
         E[c] = ex;
         E[cells + c] = ey;
@@ -524,8 +536,8 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
         float hy = oy - S * (dzx - dxz);
         float hz = oz - S * (dxy - dyx);
         if (src_here && x == a.source_x) {
-          hy -= (a.src_e[2 * P + o] * wave_y) * S;
-          hz += (a.src_e[P + o] * wave_z) * S;
+          hy -= (src_hy * wave_y) * S;
+          hz += (src_hz * wave_z) * S;
         }
         H[c] = hx;
         H[cells + c] = hy;
@@ -550,15 +562,16 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
             float* d = st.buf + (size_t)slot * 9 * cnt +
                        ((x - st.x0) * (st.y1 - st.y0) + (y - st.y0)) * (st.z1 - st.z0) +
                        (z - st.z0);
-            d[0] = ex;
-            d[cnt] = ey;
-            d[2 * cnt] = ez;
-            d[3 * cnt] = hx;
-            d[4 * cnt] = hy;
-            d[5 * cnt] = hz;
-            d[6 * cnt] = ox;
-            d[7 * cnt] = oy;
-            d[8 * cnt] = oz;
+            // streaming stores: the ring is read once, by the replay
+            __builtin_nontemporal_store(ex, d);
+            __builtin_nontemporal_store(ey, d + cnt);
+            __builtin_nontemporal_store(ez, d + 2 * cnt);
+            __builtin_nontemporal_store(hx, d + 3 * cnt);
+            __builtin_nontemporal_store(hy, d + 4 * cnt);
+            __builtin_nontemporal_store(hz, d + 5 * cnt);
+            __builtin_nontemporal_store(ox, d + 6 * cnt);
+            __builtin_nontemporal_store(oy, d + 7 * cnt);
+            __builtin_nontemporal_store(oz, d + 8 * cnt);
           }
         }
         ex = xp_x;
