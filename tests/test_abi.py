@@ -35,3 +35,24 @@ def test_bad_arguments_raise_value_error_without_a_device(pkg):
     plan = pkg._lib.YeePlan()
     with pytest.raises(ValueError, match="grid dimensions"):
         pkg._lib.call("wfd_yee_step", ctypes.byref(plan), None)
+
+
+def test_inject_many_rejects_bad_tables_without_a_device(pkg):
+    """wfd_adjoint_inject_many checks its plane table on the host: the plane
+    count, null arrays and plane shapes, before any launch."""
+    import pytest
+
+    lib = pkg._lib
+    fake = ctypes.c_void_p(0x1000)  # never dereferenced: every case fails first
+    step = ctypes.c_void_p(0x2000)
+    table = (lib.InjectSource * (lib.WFD_MAX_INJECT + 1))()
+    with pytest.raises(ValueError, match="planes"):
+        lib.call("wfd_adjoint_inject_many", ctypes.cast(table, ctypes.c_void_p),
+                 lib.WFD_MAX_INJECT + 1, fake, fake, step, 8, 8, 8, 0, None)
+    with pytest.raises(ValueError, match="arrays"):
+        lib.call("wfd_adjoint_inject_many", ctypes.cast(table, ctypes.c_void_p), 1, fake,
+                 fake, step, 8, 8, 8, 0, None)
+    table[0] = lib.InjectSource(0x3000, 0x4000, 2, 1, 1, 3, 4, 13, 5)  # 13 % (3 * 4) != 0
+    with pytest.raises(ValueError, match="shape"):
+        lib.call("wfd_adjoint_inject_many", ctypes.cast(table, ctypes.c_void_p), 1, fake,
+                 fake, step, 8, 8, 8, 0, None)
