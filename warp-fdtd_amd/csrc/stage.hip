@@ -240,7 +240,9 @@ __global__ __launch_bounds__(kBlock) void k_replay(ReplayArgs args, int slots,
   // the block's frequencies are staged in LDS one tile of kTile steps at a
   // time (every thread of the block reads the same ones).
   __shared__ double wsh[kTile * kFreqGroup * 2];
-  __shared__ double part[kSeg][kFreqGroup * 2][kPairs];
+  // segment sums of half the frequencies at a time (16 KB instead of 32 KB
+  // keeps four blocks resident per CU)
+  __shared__ double part[kSeg][kFreqGroup][kPairs];
   const int groups = (r.nf + kFreqGroup - 1) / kFreqGroup;
   const int per_group = (6 * r.count + kPairs - 1) / kPairs;
   const int g = local / per_group;
@@ -282,27 +284,35 @@ __global__ __launch_bounds__(kBlock) void k_replay(ReplayArgs args, int slots,
       }
     }
   }
+  constexpr int kHalf = kFreqGroup / 2;
 #pragma unroll
-  for (int q = 0; q < kFreqGroup; ++q) {
-    part[seg][2 * q][lane] = re[q];
-    part[seg][2 * q + 1][lane] = im[q];
-  }
-  __syncthreads();
-  if (!valid || seg != 0) return;
+  for (int h = 0; h < 2; ++h) {
+    __syncthreads();  // the previous half's sums have been read
 #pragma unroll
-  for (int q = 0; q < kFreqGroup; ++q) {
-    if (q < fn) {
-      double sr = part[0][2 * q][lane], si = part[0][2 * q + 1][lane];
+    for (int q = 0; q < kHalf; ++q) {
+      part[seg][2 * q][lane] = re[h * kHalf + q];
+      part[seg][2 * q + 1][lane] = im[h * kHalf + q];
+    }
+    __syncthreads();
+    if (valid && seg == 0) {
 #pragma unroll
-      for (int k2 = 1; k2 < kSeg; ++k2) {
-        sr += part[k2][2 * q][lane];
-        si += part[k2][2 * q + 1][lane];
+      for (int q = 0; q < kHalf; ++q) {
+        const int f = h * kHalf + q;
+        if (f < fn) {
+          double sr = part[0][2 * q][lane], si = part[0][2 * q + 1][lane];
+#pragma unroll
+          for (int k2 = 1; k2 < kSeg; ++k2) {
+            sr += part[k2][2 * q][lane];
+            si += part[k2][2 * q + 1][lane];
+          }
+          double2* p =
+              reinterpret_cast<double2*>(r.phasor64) + ((size_t)(f0 + f) * 6 * r.count + ci);
+          double2 cur = *p;
+          cur.x += sr;
+          cur.y += si;
+          *p = cur;
+        }
       }
-      double2* p = reinterpret_cast<double2*>(r.phasor64) + ((size_t)(f0 + q) * 6 * r.count + ci);
-      double2 cur = *p;
-      cur.x += sr;
-      cur.y += si;
-      *p = cur;
     }
   }
 }
