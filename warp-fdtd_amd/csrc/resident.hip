@@ -245,6 +245,9 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
   const int ncol = vy * vz;
   const bool active = tid < ncol;
   const int nb = vy + vz - 1, w1 = vz - 1;
+  // the waves holding face columns (they also poll the neighbours' faces)
+  // issue ahead of the others: the barrier that ends each half waits on them
+  if ((tid & ~63) < nb) __builtin_amdgcn_s_setprio(2);
   int eyl, ezl, hyl, hzl;
   if (tid < vz) {
     eyl = 0;
