@@ -137,7 +137,11 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
   __shared__ int bail;
   const int tid = threadIdx.x;
   const int wg = blockIdx.x;
-  const int bzi = wg % a.gz, byi = (wg / a.gz) % a.gy, bxi = wg / (a.gz * a.gy);
+  // x fastest: workgroups are dealt to the XCDs round-robin, so consecutive
+  // indices along x spread the CPML slabs (whose psi traffic loads the L2)
+  // over all XCDs; with z or y fastest each y row of sub-boxes shared one
+  // XCD, and the two CPML rows loaded two of the eight L2s (8 % slower)
+  const int bxi = wg % a.gx, byi = (wg / a.gx) % a.gy, bzi = wg / (a.gx * a.gy);
   const int X0 = bxi * a.sx, Y0 = byi * a.sy, Z0 = bzi * a.sz;
   const int vx = min(a.sx, a.nx - X0), vy = min(a.sy, a.ny - Y0), vz = min(a.sz, a.nz - Z0);
   const int sx = a.sx, sy = a.sy, sz = a.sz;
@@ -149,10 +153,10 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
   float* IE = lds + 6 * cells;    // [cells]
   float* co = lds + 7 * cells;    // [2][R][3]: E rows then H rows (x | y | z)
   const int ny = a.ny, nz = a.nz, P = ny * nz, n = a.n;
-  const int sxy = a.gz * a.gy;
-  const int wxl = bxi > 0 ? wg - sxy : -1, wxh = bxi + 1 < a.gx ? wg + sxy : -1;
-  const int wyl = byi > 0 ? wg - a.gz : -1, wyh = byi + 1 < a.gy ? wg + a.gz : -1;
-  const int wzl = bzi > 0 ? wg - 1 : -1, wzh = bzi + 1 < a.gz ? wg + 1 : -1;
+  const int sxy = a.gx * a.gy;
+  const int wxl = bxi > 0 ? wg - 1 : -1, wxh = bxi + 1 < a.gx ? wg + 1 : -1;
+  const int wyl = byi > 0 ? wg - a.gx : -1, wyh = byi + 1 < a.gy ? wg + a.gx : -1;
+  const int wzl = bzi > 0 ? wg - sxy : -1, wzh = bzi + 1 < a.gz ? wg + sxy : -1;
   const size_t per = face_words(sx, sy, sz);
   const size_t half = per / 2;
   Word* myface = a.face + (size_t)wg * per;
