@@ -41,7 +41,7 @@ HBM_PEAK_GBS = 8000.0
 # profiles/hbm_traffic_waveguide_160x96x80.md; keyed by (shape, resident,
 # steps per launch); None for others.
 MEASURED_TRAFFIC = {(SHAPE, True, 64): 47.03e6, (SHAPE, True, 256): 46.13e6,
-                    (SHAPE, True, 1000): 45.80e6, (SHAPE, False, 1): 130.9e6}
+                    (SHAPE, True, 1000): 42.72e6, (SHAPE, False, 1): 130.9e6}
 
 
 def algorithmic_bytes_per_cell(shape, pml):
