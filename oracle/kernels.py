@@ -32,6 +32,8 @@ def lib():
         if not LIB_PATH.exists():
             build()
         _lib = ctypes.CDLL(str(LIB_PATH))
+        _lib.ora_set_threads.argtypes = [_I]
+        _lib.ora_set_threads.restype = _I
         for suf, real in (("f32", ctypes.c_float), ("f64", ctypes.c_double)):
             R = real
             sig = {
@@ -58,6 +60,11 @@ def lib():
                 fn.argtypes = args
                 fn.restype = R if name == "epsilon_normal" else None
     return _lib
+
+
+def set_threads(threads):
+    """OpenMP threads of the oracle's loops; returns the previous count."""
+    return int(lib().ora_set_threads(int(threads)))
 
 
 def _suffix(a):

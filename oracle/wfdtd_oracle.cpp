@@ -13,6 +13,8 @@
 #include <cstring>
 #include <algorithm>
 
+#include <omp.h>
+
 namespace {
 
 template <typename T>
@@ -233,6 +235,9 @@ void record_dft(const T* e, const T* h, const T* prev, int t, const double* weig
                 int sx, int sy, int sz, int exact) {
   const int64_t n = (int64_t)nx * ny * nz;
   const int64_t count = (int64_t)sx * sy * sz;
+  // every (frequency, component, cell) accumulator belongs to one iteration:
+  // the parallel loop adds the same terms in the same order as a serial one
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < count; ++i) {
     int x = x0 + (int)(i / ((int64_t)sy * sz));
     int y = y0 + (int)((i / sz) % sy);
@@ -773,5 +778,13 @@ int ora_version() { return 1; }
 
 ORA_INSTANTIATE(float, f32)
 ORA_INSTANTIATE(double, f64)
+
+// OpenMP threads of the oracle's parallel loops (the CPU baseline's 1-thread
+// and all-core figures); returns the previous setting.
+int ora_set_threads(int threads) {
+  const int previous = omp_get_max_threads();
+  if (threads > 0) omp_set_num_threads(threads);
+  return previous;
+}
 
 }  // extern "C"

@@ -25,7 +25,8 @@ def copy(src, dst):
 def main(root):
     res = Path(root) / "results"
     # Input bundles (manifest + inputs.npz), loadable by Solver(case).
-    for case in ("forward50_gpu", "hann_200fs", "gradient20_highest_exact"):
+    for case in ("forward50_gpu", "hann_200fs", "gradient20_highest_exact", "forward10_gpu",
+                 "hann_10nm", "bend_flow20"):
         for name in ("manifest.json", "inputs.npz"):
             copy(res / "warp_reference" / case / name, HERE / "bundles" / case / name)
 
@@ -48,6 +49,22 @@ def main(root):
     (out / "gradient20/result.json").write_text(json.dumps(
         {k: r[k] for k in ("measurements", "finite_difference", "relative_l2", "cosine",
                            "history_bytes", "median_seconds")}, indent=1))
+    # 24M-cell long-horizon runs (400x400x150 at 10 nm): Warp mid-z field slices
+    # after 10,490 (forward10_gpu) and 15,735 (hann10) steps, and hann10's
+    # modal measurements.
+    for case in ("forward10_gpu", "hann10"):
+        copy(res / f"warp/{case}/fields.npz", out / f"{case}/warp_fields_midz.npz")
+        r = json.loads((res / f"warp/{case}/result.json").read_text())
+        (out / f"{case}/measurements.json").write_text(
+            json.dumps({"warp": r["measurements"], "relative_l2": r["relative_l2"],
+                        "steps": r["steps"], "shape": r["shape"]}, indent=1))
+    # bend_flow20: the flow-plane PhasorDetector (6, 200, 200, 1) at z = 37,
+    # mid-z slices and measurements of the 7,868-step run.
+    copy(res / "warp/bend_flow20/flow_phasors.npz", out / "bend_flow20/flow_phasors.npz")
+    copy(res / "warp/bend_flow20/fields.npz", out / "bend_flow20/warp_fields_midz.npz")
+    r = json.loads((res / "warp/bend_flow20/result.json").read_text())
+    (out / "bend_flow20/measurements.json").write_text(
+        json.dumps({"warp": r["measurements"], "relative_l2": r["relative_l2"]}, indent=1))
     # DrivenUniform observer samples (u050_small: 0.05 spacing, 462 steps).
     copy(res / "octree_guide_controls/u050_small/fields.npz", out / "u050_small/fields.npz")
     r = json.loads((res / "octree_guide_controls/u050_small/result.json").read_text())

@@ -200,3 +200,83 @@ def test_oracle_driven_guide_matches_warp_u050():
         hsamp = np.sum(hw * (0.5 * (centred[hi] + old[hi])), axis=1).reshape(3, -1)
         got = np.concatenate([esamp, hsamp])
         assert rel_l2(got, want["fields"][k]) < 1e-4
+
+
+def _plane_fields(seed, shape):
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(shape))
+    return [rng.normal(size=(3, n)).astype(np.float32) for _ in range(3)]
+
+
+@pytest.mark.parametrize("exact", [0, 1])
+def test_oracle_record_dft_single_frequency_equals_record(exact):
+    """The multi-frequency FP64 DFT with one frequency equals the reference's
+    single-frequency phasor record (detectors.py:65-106) on the same fields
+    and weights, up to the float32 accumulation of the latter; a swapped
+    real/imaginary weight or a wrong component order fails."""
+    shape, lo, sub = (6, 7, 8), (2, 1, 0), (1, 6, 8)
+    count = int(np.prod(sub))
+    rng = np.random.default_rng(5)
+    steps = 12
+    w32 = rng.normal(size=(steps, 2)).astype(np.float32)
+    ph32 = np.zeros((6, count, 2), np.float32)
+    ph64 = np.zeros((1, 6, count, 2), np.float64)
+    indices = np.arange(steps, dtype=np.int32)
+    for t in range(steps):
+        e, h, hp = _plane_fields(100 + t, shape)
+        OK.record(e, h, hp, np.array([t], np.int32), w32, indices, ph32,
+                  np.zeros(1, np.float32), np.ones(count, np.float32), shape, lo, sub, 0,
+                  exact, 1)
+        OK.record_dft(e, h, hp, t, w32.astype(np.float64)[:, None, :], ph64, shape, lo, sub,
+                      exact)
+    assert rel_l2(ph64[0], ph32) < 2e-6
+    swapped = ph64[0][..., ::-1]
+    assert rel_l2(swapped, ph32) > 0.1
+    assert rel_l2(ph64[0][[1, 0, 2, 3, 4, 5]], ph32) > 0.1
+
+
+def test_oracle_record_dft_is_linear_per_frequency():
+    """Each frequency's accumulator depends only on its own weights: three
+    frequencies at once equal three single-frequency runs bit for bit, and
+    static fields accumulate to (sum of weights) x (the colocated value)."""
+    shape, lo, sub = (5, 6, 7), (1, 0, 0), (1, 6, 7)
+    count = int(np.prod(sub))
+    rng = np.random.default_rng(9)
+    steps, nf = 9, 3
+    w = rng.normal(size=(steps, nf, 2))
+    e, h, hp = _plane_fields(3, shape)
+    joint = np.zeros((nf, 6, count, 2))
+    single = [np.zeros((1, 6, count, 2)) for _ in range(nf)]
+    for t in range(steps):
+        OK.record_dft(e, h, hp, t, w, joint, shape, lo, sub, 1)
+        for f in range(nf):
+            OK.record_dft(e, h, hp, t, np.ascontiguousarray(w[:, f:f + 1]), single[f], shape,
+                          lo, sub, 1)
+    for f in range(nf):
+        np.testing.assert_array_equal(joint[f], single[f][0])
+    # static fields: one step with weight (1, 0) gives the colocated values
+    unit = np.zeros((1, 6, count, 2))
+    OK.record_dft(e, h, hp, 0, np.array([[[1.0, 0.0]]]), unit, shape, lo, sub, 1)
+    v = unit[0, :, :, 0]
+    total = w.sum(axis=0)  # (nf, 2)
+    want = v[None, :, :, None] * total[:, None, None, :]
+    assert rel_l2(joint, want) < 1e-13
+
+
+def test_frequency_estimator_closed_form(pkg):
+    """The dispersion check's estimator recovers the frequency and decay of
+    synthetic single-frequency traces (any phase and offset) to rounding,
+    and refuses a trace that does not oscillate."""
+    PERIOD, fit_frequency = pkg.convergence.PERIOD, pkg.convergence.fit_frequency
+
+    dt = PERIOD / 41.7
+    t = np.arange(1, 1501) * dt
+    for ratio, phase, offset in ((0.95, 0.3, 0.0), (1.0, -1.2, 0.2), (1.04, 2.0, -0.1)):
+        s = np.cos(2 * np.pi * ratio * t / PERIOD + phase) + offset
+        got, residual, decay = fit_frequency(s, dt)
+        assert abs(got / ratio - 1) < 1e-12 and residual < 1e-10 and decay == 0
+        damped = np.exp(-0.03 * t / PERIOD) * (s - offset) + offset
+        got, residual, decay = fit_frequency(damped, dt, damped=True)
+        assert abs(got / ratio - 1) < 1e-10 and abs(decay - 0.03) < 1e-9
+    with pytest.raises(ValueError):
+        fit_frequency(np.exp(t / PERIOD), dt)

@@ -5,7 +5,11 @@ missing or unloadable library raises ImportError/OSError at first use, so the
 product path never runs anywhere but on the HIP kernels.
 """
 
+import atexit
 import ctypes
+import itertools
+import sys
+import weakref
 from pathlib import Path
 
 import torch
@@ -212,3 +216,98 @@ def ptr(a):
     if not t.is_contiguous():
         raise ValueError("libwfdtd arguments must be contiguous")
     return ctypes.c_void_p(t.data_ptr())
+
+
+class Handles:
+    """The HIP graphs and streams one Python object (a solver, a batch, a
+    driven guide) created through this library, destroyed by ``close()``.
+
+    Every live set is also closed by an atexit hook (``shutdown``) that runs
+    before the interpreter tears its objects down and before the HIP runtime
+    (and any profiler attached to it) runs its own exit handlers: graph
+    executables and streams left for the runtime's static destructors would
+    otherwise be released after a profiling tool has already finalized.
+    """
+
+    _live = weakref.WeakValueDictionary()
+    _order = itertools.count()
+
+    def __init__(self):
+        self._graphs = {}
+        self._stream = None
+        self.closed = False
+        Handles._live[next(Handles._order)] = self
+
+    def stream(self):
+        """The owner's capture stream (created on first use)."""
+        if self.closed:
+            raise RuntimeError("library handles already closed")
+        if self._stream is None:
+            handle = ctypes.c_void_p()
+            call("wfd_stream_create", ctypes.byref(handle))
+            self._stream = handle
+        return self._stream
+
+    def begin(self):
+        """Start capturing launches on the owner's stream."""
+        s = self.stream()
+        call("wfd_graph_begin", s)
+        return s
+
+    def end(self):
+        """Finish the capture started by ``begin``; returns the graph handle."""
+        graph = ctypes.c_void_p()
+        call("wfd_graph_end", self._stream, ctypes.byref(graph))
+        self._graphs[graph.value] = graph
+        return graph
+
+    def destroy(self, graph):
+        """Destroy one graph of this owner (no-op for unknown handles)."""
+        if graph is not None and self._graphs.pop(graph.value, None) is not None:
+            call("wfd_graph_destroy", graph)
+
+    def close(self):
+        """Destroy every graph, then the stream; idempotent."""
+        if self.closed:
+            return
+        self.closed = True
+        graphs, self._graphs = list(self._graphs.values()), {}
+        for g in graphs:
+            call("wfd_graph_destroy", g)
+        if self._stream is not None:
+            s, self._stream = self._stream, None
+            call("wfd_stream_destroy", s)
+
+
+def release(handles):
+    """``close()`` for an owner's ``__del__``: skipped while the interpreter
+    is finalizing (the atexit hook has closed every live set by then) and
+    silent on failure (a destructor must not raise)."""
+    if handles is None or sys.is_finalizing():
+        return
+    try:
+        handles.close()
+    except Exception:
+        pass
+
+
+def shutdown():
+    """Close every live handle set, newest first, after the device drained."""
+    if _lib is None:
+        return
+    live = sorted(Handles._live.items(), key=lambda kv: -kv[0])
+    if not live:
+        return
+    try:
+        if torch.cuda.is_initialized():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+    for _, handles in live:
+        try:
+            handles.close()
+        except Exception:
+            pass
+
+
+atexit.register(shutdown)

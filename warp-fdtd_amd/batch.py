@@ -12,13 +12,12 @@ captures one HIP graph per chunk of ``graph_steps`` steps that steps every
 design and replays every design's staged detectors; ``run`` replays them.
 """
 
-import ctypes
 import time
 
 import numpy as np
 import torch
 
-from ._lib import WFD_MAX_BATCH, YeePlan, call, ptr, stream
+from ._lib import WFD_MAX_BATCH, Handles, YeePlan, call, ptr, release, stream
 from .array import Array
 
 
@@ -40,7 +39,7 @@ class Batch:
         self.graph_steps = graph_steps
         self.device = self.solvers[0].device
         self.graphs, self.quiet_graphs = {}, {}
-        self._stream = None
+        self._handles = Handles()
         self._tables = {}
         self.cells = sum(s.n for s in self.solvers)
 
@@ -61,14 +60,9 @@ class Batch:
                 s._replay_staged(1)
 
     def _capture(self, length, record):
-        if self._stream is None:
-            handle = ctypes.c_void_p()
-            call("wfd_stream_create", ctypes.byref(handle))
-            self._stream = handle
-        s = self._stream
         plans, ws = self._tables[record]
         torch.cuda.synchronize(self.device)
-        call("wfd_graph_begin", s)
+        s = self._handles.begin()
         ext = torch.cuda.ExternalStream(s.value, device=self.device)
         try:
             with torch.cuda.stream(ext):
@@ -82,8 +76,7 @@ class Batch:
                             solver._replay_staged(m)
                     done += m
         finally:
-            graph = ctypes.c_void_p()
-            call("wfd_graph_end", s, ctypes.byref(graph))
+            graph = self._handles.end()
         return graph
 
     # ------------------------------------------------------------ driver
@@ -149,19 +142,35 @@ class Batch:
             t += length
         torch.cuda.synchronize(self.device)
 
+    def run_steps(self, count):
+        """Advance every design exactly `count` steps from its current step
+        (recorded graphs of at most graph_steps; missing lengths are captured
+        on first use) and synchronize."""
+        if not self.graphs:
+            raise RuntimeError("compile() must run before run_steps()")
+        left = int(count)
+        while left > 0:
+            length = min(self.graph_steps, left)
+            if length not in self.graphs:
+                self.graphs[length] = self._capture(length, True)
+            call("wfd_graph_launch", self.graphs[length], stream())
+            left -= length
+        torch.cuda.synchronize(self.device)
+
     def _destroy_graphs(self):
         for table in (self.graphs, self.quiet_graphs):
             for g in table.values():
-                call("wfd_graph_destroy", g)
+                self._handles.destroy(g)
             table.clear()
 
+    def close(self):
+        """Destroy the batch's HIP graphs and capture stream (idempotent)."""
+        self.graphs.clear()
+        self.quiet_graphs.clear()
+        self._handles.close()
+
     def __del__(self):
-        try:
-            self._destroy_graphs()
-            if self._stream is not None:
-                call("wfd_stream_destroy", self._stream)
-        except Exception:
-            pass
+        release(getattr(self, "_handles", None))
 
 
 def call_bytes(count):

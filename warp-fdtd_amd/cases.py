@@ -308,10 +308,16 @@ def batch_shapes(count=32, seed=0, lo=(120, 120, 64), hi=(256, 224, 96), pml=10)
 
 
 def _batch_member(args):
+    """One bundle of the batch, in a worker process: its BLAS runs on one
+    thread (the pool already occupies the cores; threaded ARPACK vectors in
+    every worker oversubscribe them several times over)."""
+    from threadpoolctl import threadpool_limits
+
     path, shape, side, blurred, seed, steps, pml, wavelength, flux_wavelengths = args
     rho = design_density(np.random.default_rng(seed), (side, side), blurred)
-    bend_case(path, rho, shape=shape, steps=steps, pml=pml, design=(side, side),
-              wavelength=wavelength, flux_wavelengths=flux_wavelengths)
+    with threadpool_limits(limits=1):
+        bend_case(path, rho, shape=shape, steps=steps, pml=pml, design=(side, side),
+                  wavelength=wavelength, flux_wavelengths=flux_wavelengths)
     return str(path), rho
 
 

@@ -13,7 +13,7 @@ import time
 import numpy as np
 import torch
 
-from ._lib import GuideDesc, call, ptr, stream
+from ._lib import GuideDesc, Handles, call, ptr, release, stream
 from .array import Array
 
 
@@ -47,7 +47,7 @@ class DrivenUniform:
         self.arrays = [self.e, self.h, self.counter, self.wave]
         self.graph = None
         self.graph_steps = 32
-        self._capture_stream = None
+        self._handles = Handles()
         p = problem
         g = GuideDesc()
         g.nx, g.ny, g.nz = self.shape
@@ -87,23 +87,16 @@ class DrivenUniform:
         self.step()
         torch.cuda.synchronize(self.device)
         self.reset()
-        if self._capture_stream is None:
-            handle = ctypes.c_void_p()
-            call("wfd_stream_create", ctypes.byref(handle))
-            self._capture_stream = handle
-        s = self._capture_stream
         torch.cuda.synchronize(self.device)
-        call("wfd_graph_begin", s)
+        s = self._handles.begin()
         ext = torch.cuda.ExternalStream(s.value, device=self.device)
         try:
             with torch.cuda.stream(ext):
                 for _ in range(self.graph_steps):
                     self.step()
         finally:
-            graph = ctypes.c_void_p()
-            call("wfd_graph_end", s, ctypes.byref(graph))
-        if self.graph is not None:
-            call("wfd_graph_destroy", self.graph)
+            graph = self._handles.end()
+        self._handles.destroy(self.graph)
         self.graph = graph
         self.reset()
         return time.perf_counter() - start
@@ -121,11 +114,10 @@ class DrivenUniform:
                 self.step()
         torch.cuda.synchronize(self.device)
 
+    def close(self):
+        """Destroy the captured graph and stream (idempotent)."""
+        self.graph = None
+        self._handles.close()
+
     def __del__(self):
-        try:
-            if self.graph is not None:
-                call("wfd_graph_destroy", self.graph)
-            if self._capture_stream is not None:
-                call("wfd_stream_destroy", self._capture_stream)
-        except Exception:
-            pass
+        release(getattr(self, "_handles", None))

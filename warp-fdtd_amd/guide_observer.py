@@ -21,25 +21,9 @@ def points(problem):
     return axes, np.stack(np.meshgrid(*axes, indexing="ij"), axis=-1).reshape(-1, 3)
 
 
-def stencil(xyz, origin, h, shape, component, electric, offset):
-    """guide_observer.py:37-49 — trilinear corner indices and weights of one
-    staggered component grid at point xyz."""
-    staggering = np.zeros(3) if electric else np.full(3, 0.5)
-    staggering[component] = 0.5 if electric else 0
-    index = (xyz - origin) / h - staggering
-    base = np.clip(np.floor(index).astype(int), 0, np.array(shape) - 2)
-    fraction = index - base
-    indices = []
-    weights = []
-    for corner in product((0, 1), repeat=3):
-        p = base + corner
-        indices.append(offset + np.ravel_multi_index(tuple(p), shape))
-        weights.append(np.prod(np.where(corner, fraction, 1 - fraction)))
-    return indices, weights
-
-
 def _stencils(xyz, h, shape, component, electric, offset):
-    """stencil() for many points at once (same arithmetic, vectorised)."""
+    """Trilinear corner indices and weights of one staggered component grid
+    (guide_observer.py:37-49) for many points at once."""
     staggering = np.zeros(3) if electric else np.full(3, 0.5)
     staggering[component] = 0.5 if electric else 0
     index = xyz / h - staggering

@@ -34,27 +34,6 @@ class GuideProblem:
     def lengths(self):
         return (self.length, self.width, self.height)
 
-    def sample(self, coords, h):
-        """Aligned-box volume fraction (guide_problem.py:35-56)."""
-        fractions = []
-        for axis, width, center in ((1, self.core_width, self.width / 2),
-                                    (2, self.core_height, self.height / 2)):
-            fractions.append(np.clip(
-                (np.minimum(coords[:, axis] + h / 2, center + width / 2)
-                 - np.maximum(coords[:, axis] - h / 2, center - width / 2)) / h, 0, 1))
-        return self.clad_epsilon + (self.core_epsilon - self.clad_epsilon) * fractions[0] * \
-            fractions[1]
-
-    def source(self, coords, components):
-        delta = (coords - [self.source_x, self.width / 2, self.height / 2]) / [
-            self.source_sigma_x, self.source_sigma_transverse, self.source_sigma_transverse]
-        return np.exp(-0.5 * np.sum(delta ** 2, axis=1)) * (components == 1)
-
-    def damping(self, coords):
-        distance = np.minimum(coords, np.array(self.lengths) - coords)
-        depth = np.maximum(0, 1 - distance / self.sponge_width)
-        return self.sponge_rate * np.sum(depth ** 3, axis=1)
-
     def waveform(self, t):
         """guide_problem.py:72-75"""
         return np.exp(-0.5 * ((t - self.pulse_center) / self.pulse_sigma) ** 2) * np.sin(

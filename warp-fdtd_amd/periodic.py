@@ -4,14 +4,13 @@ periodic=1, unit permittivity and inactive CPML rows, and records E_x of the
 first cell every step.  The fourth-order RK4 branch of the reference is a
 separate, non-Yee method and is not part of this package."""
 
-import ctypes
 import time
 
 import numpy as np
 import torch
 
 from . import kernels as k
-from ._lib import call, ptr, stream
+from ._lib import Handles, call, ptr, release, stream
 from .array import Array
 
 
@@ -45,7 +44,7 @@ class Periodic:
         self.coeff = Array.from_numpy(coeff, np.float32, self.device)
         self.dummy = Array.zeros((3, 1), device=self.device)
         self.graphs = {}
-        self._capture_stream = None
+        self._handles = Handles()
 
     def launch_step(self):
         """periodic.py:136-188 (Yee branch)."""
@@ -66,21 +65,18 @@ class Periodic:
         self.launch_step()
         torch.cuda.synchronize(self.device)
         self.reset()
-        if self._capture_stream is None:
-            handle = ctypes.c_void_p()
-            call("wfd_stream_create", ctypes.byref(handle))
-            self._capture_stream = handle
-        s = self._capture_stream
+        for g in self.graphs.values():
+            self._handles.destroy(g)
+        self.graphs = {}
         for length in {64, self.steps % 64} - {0}:
-            call("wfd_graph_begin", s)
+            s = self._handles.begin()
             ext = torch.cuda.ExternalStream(s.value, device=self.device)
             try:
                 with torch.cuda.stream(ext):
                     for _ in range(length):
                         self.launch_step()
             finally:
-                graph = ctypes.c_void_p()
-                call("wfd_graph_end", s, ctypes.byref(graph))
+                graph = self._handles.end()
             self.graphs[length] = graph
         self.reset()
         return time.perf_counter() - start
@@ -92,3 +88,11 @@ class Periodic:
         if self.steps % 64:
             call("wfd_graph_launch", self.graphs[self.steps % 64], s)
         torch.cuda.synchronize(self.device)
+
+    def close(self):
+        """Destroy the captured graphs and stream (idempotent)."""
+        self.graphs = {}
+        self._handles.close()
+
+    def __del__(self):
+        release(getattr(self, "_handles", None))

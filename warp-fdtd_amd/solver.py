@@ -19,7 +19,8 @@ import numpy as np
 import torch
 
 from . import kernels as k
-from ._lib import ReplayDesc, ResidentLayout, WFD_MAX_STAGES, YeePlan, call, ptr, stream
+from ._lib import (Handles, ReplayDesc, ResidentLayout, WFD_MAX_STAGES, YeePlan, call, ptr,
+                   release, stream)
 from .array import Array
 
 
@@ -241,7 +242,7 @@ class Solver:
         self.save_h_from = self.steps
         self.graphs = {}
         self._quiet_graphs = {}
-        self._capture_stream = None
+        self._handles = Handles()
         self._flush_every = 1
         self._pending = 0
         self.resident_mode = resident
@@ -451,15 +452,10 @@ class Solver:
     def _capture_fn(self, length, fn):
         """HIP graph of `length` calls of fn, captured on a library stream;
         staged monitors replay every graph_steps steps and at the end."""
-        if self._capture_stream is None:
-            handle = ctypes.c_void_p()
-            call("wfd_stream_create", ctypes.byref(handle))
-            self._capture_stream = handle
-        s = self._capture_stream
         torch.cuda.synchronize(self.device)
         # the stage rings hold graph_steps steps: longer graphs flush inside
         self._flush_every, self._pending = min(length, self.graph_steps), 0
-        call("wfd_graph_begin", s)
+        s = self._handles.begin()
         ext = torch.cuda.ExternalStream(s.value, device=self.device)
         try:
             with torch.cuda.stream(ext):
@@ -468,8 +464,7 @@ class Solver:
                 if self._pending:
                     self._replay_staged(self._pending)
         finally:
-            graph = ctypes.c_void_p()
-            call("wfd_graph_end", s, ctypes.byref(graph))
+            graph = self._handles.end()
             self._flush_every, self._pending = 1, 0
         return graph
 
@@ -523,7 +518,7 @@ class Solver:
         return [self.graph_steps] * (self.steps // self.graph_steps) + (
             [self.steps % self.graph_steps] if self.steps % self.graph_steps else [])
 
-    def _run_resident(self):
+    def _run_resident(self, events=None):
         """Chunks with an active detector window run as recorded launches of
         graph_steps; runs of quiet chunks merge into one launch."""
         active = self._windows()
@@ -531,22 +526,23 @@ class Solver:
         for length in self._chunks():
             if self.monitors and active[t:t + length].any():
                 if quiet:
-                    self.resident_steps(quiet, record=False)
+                    self.resident_steps(quiet, record=False, events=events)
                     quiet = 0
-                self.resident_steps(length, record=True)
+                self.resident_steps(length, record=True, events=events)
             else:
                 quiet += length
             t += length
         if quiet:
-            self.resident_steps(quiet, record=False)
+            self.resident_steps(quiet, record=False, events=events)
         self.check()
 
-    def run(self):
-        """solver.py:252-258 — replay the captured graphs over all steps."""
+    def run(self, events=None):
+        """solver.py:252-258 — replay the captured graphs over all steps.
+        `events` (a list, resident stepping only): see resident_steps."""
         if not self._compiled:
             raise RuntimeError("compile() must run before run()")
         if self.resident:
-            self._run_resident()
+            self._run_resident(events)
             return
         active = self._windows()
         s = stream()
@@ -594,13 +590,17 @@ class Solver:
     def _destroy_graphs(self):
         for table in (self.graphs, self._quiet_graphs):
             for g in table.values():
-                call("wfd_graph_destroy", g)
+                self._handles.destroy(g)
             table.clear()
 
+    def close(self):
+        """Destroy this solver's HIP graphs and capture stream (idempotent;
+        also done at interpreter exit).  The solver cannot replay graphs
+        afterwards."""
+        self.graphs.clear()
+        self._quiet_graphs.clear()
+        self._compiled = False
+        self._handles.close()
+
     def __del__(self):
-        try:
-            self._destroy_graphs()
-            if self._capture_stream is not None:
-                call("wfd_stream_destroy", self._capture_stream)
-        except Exception:
-            pass
+        release(getattr(self, "_handles", None))
