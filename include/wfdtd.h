@@ -125,6 +125,37 @@ typedef struct wfd_yee_plan {
 
 int wfd_yee_step(const wfd_yee_plan* plan, void* stream);
 
+/* The same step (bitwise) as one launch that reads the plan's e, h, psi_e
+ * and psi_h (the state after counters[0] steps) and writes the next state to
+ * the targets (buffers of the same shapes, distinct from the plan's); the
+ * caller alternates the two sets.  With steps = 2 the launch applies two
+ * steps (the first step's fields stay on chip; counters advance by 2, both
+ * steps are staged, and hprev gets H at the start of the second step).  Each workgroup recomputes the new E of the
+ * row past its tile and of the plane past its chunk instead of waiting for
+ * the workgroups that own them, so every cell's E, H and 1/eps are read once
+ * and written once per step (the fused pair of wfd_yee_step reads E and H
+ * twice).  Same step protocol as wfd_yee_step (counters, staging, hprev,
+ * source); x_chunk is not used.  Only psi components on a CPML axis of a
+ * cell are written to the targets (the others are neither read nor written).
+ * The ticket is a device int that is zero before the first call and left
+ * zero by every call; calls sharing a ticket must not overlap.  No reference
+ * counterpart (replaces the same pair of launches as wfd_yee_step). */
+typedef struct wfd_yee_targets {
+  float* e;
+  float* h;
+  float* psi_e;
+  float* psi_h;
+  int* ticket;
+  int planes;           /* x planes per workgroup (0: chosen) */
+  int slots_per_thread; /* one step: 2 or 4 (0: 4), a 256-thread workgroup
+                           covering 256 x this - nz (y, z) cells; two steps:
+                           1 or 2 (0: 2), 512 threads covering 512 x this -
+                           3 nz cells */
+  int steps;            /* steps per launch: 1 or 2 (0: 1) */
+} wfd_yee_targets;
+
+int wfd_yee_pass(const wfd_yee_plan* plan, const wfd_yee_targets* out, void* stream);
+
 /* ------------------------------------------------------------ batched grids
  * Several independent grids (one wfd_yee_plan each, any shapes) stepped by
  * one launch per half-step and x_chunk class: the launch concatenates every
@@ -151,9 +182,13 @@ int wfd_batch_step(const wfd_yee_plan* plans, int count, const void* workspace, 
  * half-step launches of solver.py:203-234); counters advance by nsteps.
  * layout_for validates the decomposition (WFD_ERR_UNSUPPORTED when the
  * workgroups cannot all be co-resident) and sizes the workspace (8-byte
- * aligned, allocated by the caller); the int after the face words is an
- * error word, nonzero after a bounded poll gave up (the fields of that run
- * are then invalid), which the caller reads and clears. */
+ * aligned, allocated by the caller).  The caller zeroes the workspace before
+ * its first launch and whenever the layout used with it changes; launches
+ * of one layout need no clearing between them (the face words carry tags
+ * derived from an epoch word that each launch advances).  The int after
+ * the face words is an error word, nonzero after a bounded poll gave up
+ * (the fields of that run are then invalid), which the caller reads and
+ * clears (zeroing the whole workspace); the int after it is the epoch. */
 typedef struct wfd_resident_layout {
   int sx, sy, sz;        /* sub-box cells along x, y, z */
   int gx, gy, gz;        /* workgroups along each axis */
@@ -306,6 +341,15 @@ typedef struct wfd_inject_source {
 int wfd_adjoint_inject_many(const wfd_inject_source* sources, int count, float* qe,
                             float* qh, const int* step, int nx, int ny, int nz, int old_h_only,
                             void* stream);
+/* The launches between two reverse steps of adjoint.py:388-414 in one: the
+ * H_old terms of step t = step[0] (inject(1)), the retreat, and every term
+ * of step t - 1 (the next step's inject(0)); per cell the same additions in
+ * the same order, so the result equals wfd_adjoint_inject_many(old_h_only =
+ * 1), wfd_advance(-1), wfd_adjoint_inject_many(old_h_only = 0) bit for bit.
+ * ticket: a device int, zero before the first call and left zero by each
+ * call (calls sharing it must not overlap). */
+int wfd_adjoint_inject_turn(const wfd_inject_source* sources, int count, float* qe, float* qh,
+                            int* step, int* ticket, int nx, int ny, int nz, void* stream);
 /* adjoint.py:472-490 write_design: plane (sx, sy) into every z-layer of the box. */
 int wfd_write_design(const float* plane, float* eps, int nx, int ny, int nz, int x0,
                      int y0, int z0, int sx, int sy, int sz, void* stream);

@@ -175,6 +175,26 @@ def test_odd_checkpoint_chunks_equal_full_history(pkg, gradient20):
     np.testing.assert_array_equal(gradient2, gradient)
 
 
+def test_turned_inject_equals_the_reference_launch_order(pkg, gradient20):
+    """Captured reverse steps ending with wfd_adjoint_inject_turn (inject(1),
+    retreat and the next step's inject(0) in one launch) give the gradient
+    of the reference's five launches per step bit for bit."""
+    p0, loss, gradient, voxel, meas = gradient20
+    out = []
+    for turned in (True, False):
+        adj = pkg.adjoint.Adjoint(BUNDLES / "gradient20_highest_exact", checkpoint_every=64)
+        adj.turned_inject = turned
+        pkg.detectors.attach(adj)
+        loss2, gradient2 = adj.differentiate(p0)
+        assert adj._turns() == turned
+        assert int(adj.reverse_step.numpy()[0]) == -1
+        out.append((loss2, adj.voxel_gradient()))
+        adj.close()
+    assert out[0][0] == out[1][0] == loss
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    np.testing.assert_array_equal(out[0][1], voxel)
+
+
 @pytest.mark.parametrize("from_h", [1, 0])
 def test_fused_reverse_half_equals_prepare_then_gather(pkg, gpu, from_h):
     """wfd_adjoint_reverse_half (prepare + gather in one pass, psi to a second

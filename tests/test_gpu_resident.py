@@ -15,7 +15,7 @@ from conftest import BUNDLES
 
 pytestmark = pytest.mark.gpu
 
-RESIDENT_CELLS = 38912 // 7  # LDS floats per workgroup / 7 floats per cell
+RESIDENT_CELLS = 40896 // 7  # LDS floats per workgroup (kLds) / 7 floats per cell
 
 
 def _pair(pkg, case, attach=True, **kw):

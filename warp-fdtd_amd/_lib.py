@@ -52,6 +52,13 @@ class YeePlan(ctypes.Structure):
     ]
 
 
+class YeeTargets(ctypes.Structure):
+    """wfd_yee_targets (include/wfdtd.h)."""
+
+    _fields_ = [("e", P), ("h", P), ("psi_e", P), ("psi_h", P), ("ticket", P), ("planes", I),
+                ("slots_per_thread", I), ("steps", I)]
+
+
 class InjectSource(ctypes.Structure):
     """wfd_inject_source (include/wfdtd.h)."""
 
@@ -126,6 +133,7 @@ SIGNATURES = {
     "wfd_update_h_packed": _UPDATE,
     "wfd_advance": [P, I, P],
     "wfd_yee_step": [ctypes.POINTER(YeePlan), P],
+    "wfd_yee_pass": [ctypes.POINTER(YeePlan), ctypes.POINTER(YeeTargets), P],
     "wfd_batch_workspace_bytes": [I],
     "wfd_batch_prepare": [ctypes.POINTER(YeePlan), I, P, L, P],
     "wfd_batch_step": [ctypes.POINTER(YeePlan), I, P, P],
@@ -145,6 +153,7 @@ SIGNATURES = {
     "wfd_adjoint_reverse_half_f64": [P, P, P, P, P, P, I, P, I, P, P] + [I] * 9 + [F, I, P],
     "wfd_adjoint_inject": [P, P, P, P, P] + [I] * 11 + [P],
     "wfd_adjoint_inject_many": [P, I, P, P, P, I, I, I, I, P],
+    "wfd_adjoint_inject_turn": [P, I, P, P, P, P, I, I, I, P],
     "wfd_write_design": [P, P] + [I] * 9 + [P],
     "wfd_driven_magnetic": [ctypes.POINTER(GuideDesc), P, P, P],
     "wfd_driven_electric": [ctypes.POINTER(GuideDesc), P, P, P, P, P],

@@ -148,6 +148,11 @@ class Adjoint(Solver):
         self._flip = 0
         self.grad = Array.zeros(self.design_n, np.dtype(grad_dtype), self.device)
         self.reverse_step = Array.zeros(1, np.int32, self.device)
+        # captured reverse steps end with one launch for inject(1), the retreat
+        # and the next step's inject(0) (wfd_adjoint_inject_turn); False keeps
+        # the reference's five launches per step (same bits)
+        self.turned_inject = True
+        self._turn_ticket = Array.zeros(1, np.int32, self.device)
         self.adjoint_sources = []
         self.reverse_graphs = {}
         self.checkpoints = None
@@ -166,7 +171,8 @@ class Adjoint(Solver):
         half-step, i.e. before the fused E+H step as well."""
         if curl:
             x0, y0, z0, sx, sy, sz = self.box
-            call("wfd_record_curl", ptr(self.H), ptr(self.history), int(self.history.shape[0]),
+            call("wfd_record_curl", ptr(self.current_h()), ptr(self.history),
+                 int(self.history.shape[0]),
                  ptr(self.step), *self.shape, x0, y0, z0, sx, sy, sz, float(self.courant),
                  stream())
         self._step_fields(stage=record)
@@ -179,6 +185,12 @@ class Adjoint(Solver):
         self.launch_step()
         torch.cuda.synchronize(self.device)
         self.tune()
+        self.capture_all()
+        return time.perf_counter() - start
+
+    def capture_all(self):
+        """The forward, quiet and recompute graphs with the current stepping
+        choices (compile's tail)."""
         self.reset()
         self._destroy_graphs()
         full = self.checkpoint_every is None
@@ -188,19 +200,19 @@ class Adjoint(Solver):
             lengths |= {K, self.steps % K} - {0}
         for length in lengths:
             # forward with monitors; with checkpoints the curl history is only
-            # recorded during recomputation
+            # recorded during recomputation (the curl history needs H between
+            # every two steps: one-step passes there)
             self.graphs[length] = self._capture_fn(
-                length, lambda: self.launch_step(record=True, curl=full))
+                length, lambda: self.launch_step(record=True, curl=full), two_step=not full)
             self._quiet_graphs[length] = self._capture_fn(
-                length, lambda: self.launch_step(record=False, curl=full))
+                length, lambda: self.launch_step(record=False, curl=full), two_step=not full)
         self._recompute_graphs = {}
         if not full:
             for length in {self.checkpoint_every, self.steps % self.checkpoint_every} - {0}:
                 self._recompute_graphs[length] = self._capture_fn(
-                    length, lambda: self.launch_step(record=False, curl=True))
+                    length, lambda: self.launch_step(record=False, curl=True), two_step=False)
         self.reset()
         self._compiled = True
-        return time.perf_counter() - start
 
     def _replay_graphs(self, t0, t1, graphs_by_len, quiet_by_len, active):
         """Replay graphs covering steps [t0, t1) in chunks of the largest length."""
@@ -362,10 +374,36 @@ class Adjoint(Solver):
         self.inject(1)
         _retreat(self.reverse_step)
 
+    def _turns(self):
+        """Whether the captured reverse steps use wfd_adjoint_inject_turn."""
+        return self.turned_inject and len(self.adjoint_sources) <= WFD_MAX_INJECT
+
+    def _reverse_one_turned(self):
+        """reverse_one with this step's inject(0) already applied (by the
+        previous turn, or by backward() before the first step), ending with
+        the turn: inject(1) of this step, retreat, inject(0) of the next."""
+        if self._flip:
+            e_in, e_out, h_in, h_out = self.psie2, self.psie, self.psih2, self.psih
+        else:
+            e_in, e_out, h_in, h_out = self.psie, self.psie2, self.psih, self.psih2
+        _reverse_half(self.qe, self.qh, self.eps, h_in, h_out, self.coeffs["H"], self.history,
+                      self.grad, self.reverse_step, *self.shape, *self.box, self.courant, 1)
+        _reverse_half(self.qh, self.qe, self.eps, e_in, e_out, self.coeffs["E"], self.history,
+                      self.grad, self.reverse_step, *self.shape, *self.box, self.courant, 0)
+        self._flip ^= 1
+        table = (ctypes.cast(self._inject_table(), ctypes.c_void_p) if self.adjoint_sources
+                 else None)
+        call("wfd_adjoint_inject_turn", table, len(self.adjoint_sources), ptr(self.qe),
+             ptr(self.qh), ptr(self.reverse_step), ptr(self._turn_ticket), *self.shape,
+             stream())
+
     def _reverse_chunk(self, length):
         """`length` reverse steps leaving the psi state in the first buffers."""
         for _ in range(length):
-            self.reverse_one()
+            if self._turns():
+                self._reverse_one_turned()
+            else:
+                self.reverse_one()
         if self._flip:
             self.psie.tensor.copy_(self.psie2.tensor)
             self.psih.tensor.copy_(self.psih2.tensor)
@@ -386,11 +424,13 @@ class Adjoint(Solver):
                              "reverse graphs capture the cotangent buffers")
         self.reset_reverse()
         self.reverse_one()
+        if self._turns():
+            self._reverse_one_turned()
         torch.cuda.synchronize(self.device)
         self.reset_reverse()
         for length in self._reverse_lengths():
             self.reverse_graphs[length] = self._capture_fn(
-                1, lambda length=length: self._reverse_chunk(length))
+                1, lambda length=length: self._reverse_chunk(length), stepping=False)
         self.reset_reverse()
 
     def _reverse_lengths(self):
@@ -400,6 +440,8 @@ class Adjoint(Solver):
 
     def backward(self):
         s = stream()
+        if self._turns():
+            self.inject(0)  # the first step's; every turn applies the next one's
         if self.checkpoint_every is None:
             for _ in range(self.steps // self.graph_steps):
                 call("wfd_graph_launch", self.reverse_graphs[self.graph_steps], s)

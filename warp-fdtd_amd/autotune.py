@@ -4,7 +4,8 @@ The free launch parameter of wfd_yee_step is the x chunk a thread column
 marches (1, 2, 4 or 8 planes, compile-time unrolled).  ``tune`` times every
 candidate on the solver's own grid and buffers with HIP events and returns
 the fastest; ``tune_resident`` times the on-chip-resident kernel
-(wfd_resident_run) against the tuned fused step.  Results are cached in
+(wfd_resident_run) against the tuned fused step, and ``tune_pass`` the
+single-pass step (wfd_yee_pass: x planes per workgroup, slots per thread).  Results are cached in
 memory and in a JSON file keyed by grid shape, CPML rows and device name, so
 a shape is tuned once per machine.
 """
@@ -175,6 +176,73 @@ def tune_resident(solver, use_cache=True, require=False):
     best_us = timings[f"{best.sx}x{best.sy}x{best.sz}"]
     use = best_us < fused
     entry = {"resident": use, "layout": [best.sx, best.sy, best.sz],
+             "us": {"fused": round(fused, 3), **{k: round(v, 3) for k, v in timings.items()}}}
+    _memory[key] = entry
+    table = _load()
+    table[key] = entry
+    _store(table)
+    return best if (use or require) else None
+
+
+PASS_PLANES = (1, 2, 4, 8)
+PASS_SHAPES = {1: (2, 4), 2: (1, 2)}  # steps per launch -> slots per thread
+
+
+def _pass_fits(nz, steps, slots):
+    threads = 256 if steps == 1 else 512
+    return (nz if steps == 1 else 3 * nz) < slots * threads
+
+
+def measure_pass(solver, planes, slots, steps=1, repeats=20):
+    """Median microseconds per step of wfd_yee_pass launches (a pair there
+    and back) with the given shape; runs real steps (reset after)."""
+    solver.use_single_pass(planes, slots, steps)
+    kinds = ["p2", "skip", "p2", "skip"] if steps == 2 else ["p1", "p1"]
+    try:
+        def pair():
+            solver._schedule = list(kinds)
+            for _ in kinds:
+                solver._pass_step(False)
+
+        for _ in range(3):
+            pair()
+        events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                  for _ in range(repeats)]
+        for a, b in events:
+            a.record()
+            pair()
+            b.record()
+        torch.cuda.synchronize(solver.device)
+    finally:
+        solver.single_pass = False
+        solver._phase, solver._schedule = 0, []
+    return float(np.median([a.elapsed_time(b) for a, b in events]) * 1e3 / len(kinds))
+
+
+def tune_pass(solver, use_cache=True, require=False):
+    """(planes, slots per thread, steps per launch) of the fastest single-pass
+    shape if it beats the tuned fused step on this grid (or in any case with
+    ``require``), else None."""
+    key = _key(solver) + "|single-pass-3"
+    if use_cache:
+        hit = _memory.get(key)
+        if hit is None:
+            hit = _load().get(key)
+        if hit is not None:
+            _memory[key] = hit
+            return tuple(hit["shape"]) if (hit["single_pass"] or require) else None
+    nz = solver.shape[2]
+    shapes = [(p, j, k) for k, slots in PASS_SHAPES.items() for j in slots
+              if _pass_fits(nz, k, j) for p in PASS_PLANES]
+    if not shapes:
+        if require:
+            raise ValueError(f"no single-pass shape fits nz = {nz}")
+        return None
+    fused = measure(solver, solver.x_chunk or 2)
+    timings = {"x".join(map(str, sh)): measure_pass(solver, *sh) for sh in shapes}
+    best = min(shapes, key=lambda sh: timings["x".join(map(str, sh))])
+    use = timings["x".join(map(str, best))] < fused
+    entry = {"single_pass": use, "shape": list(best),
              "us": {"fused": round(fused, 3), **{k: round(v, 3) for k, v in timings.items()}}}
     _memory[key] = entry
     table = _load()

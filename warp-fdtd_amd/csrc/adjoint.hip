@@ -297,24 +297,34 @@ struct InjectMany {
   int count;
 };
 
-__global__ __launch_bounds__(kBlock) void k_inject_many(InjectMany a, float* __restrict__ qe,
-                                                        float* __restrict__ qh,
-                                                        const int* __restrict__ step, int nx,
-                                                        int ny, int nz, int old_h_only) {
+// Target cell of one thread of k_inject_many's grid (false: none, or a cell
+// an earlier plane's thread owns).
+__device__ __forceinline__ bool inject_target(const InjectMany& a, int nx, int ny, int nz,
+                                              int& k, int& x, int& y, int& z) {
   const int b = blockIdx.x;
-  int k = 0;
+  k = 0;
   while (k + 1 < a.count && b >= a.begin[k + 1]) ++k;
+  if (a.count == 0) return false;
   const InjectSource& g = a.g[k];
   const int sx = g.count / (g.sy * g.sz);
   const int tsy = g.sy + 1, tsz = g.sz + 1;
   const int r = (b - a.begin[k]) * kBlock + threadIdx.x;
-  if (r >= (sx + 1) * tsy * tsz) return;
-  const int x = g.x0 - 1 + r / (tsy * tsz);
-  const int y = g.y0 - 1 + (r / tsz) % tsy;
-  const int z = g.z0 + r % tsz;
-  if (x < 0 || y < 0 || x >= nx || y >= ny || z >= nz) return;
+  if (r >= (sx + 1) * tsy * tsz) return false;
+  x = g.x0 - 1 + r / (tsy * tsz);
+  y = g.y0 - 1 + (r / tsz) % tsy;
+  z = g.z0 + r % tsz;
+  if (x < 0 || y < 0 || x >= nx || y >= ny || z >= nz) return false;
   for (int j = 0; j < k; ++j)
-    if (in_targets(a.g[j], x, y, z)) return;
+    if (in_targets(a.g[j], x, y, z)) return false;
+  return true;
+}
+
+__global__ __launch_bounds__(kBlock) void k_inject_many(InjectMany a, float* __restrict__ qe,
+                                                        float* __restrict__ qh,
+                                                        const int* __restrict__ step, int nx,
+                                                        int ny, int nz, int old_h_only) {
+  int k, x, y, z;
+  if (!inject_target(a, nx, ny, nz, k, x, y, z)) return;
   const int t = step[0];
   const int n = nx * ny * nz;
   const int i = (x * ny + y) * nz + z;
@@ -327,6 +337,42 @@ __global__ __launch_bounds__(kBlock) void k_inject_many(InjectMany a, float* __r
   if (!old_h_only)
     for (int c = 0; c < 3; ++c) qe[c * n + i] = e[c];
   for (int c = 0; c < 3; ++c) qh[c * n + i] = h[c];
+}
+
+// The turn between two reverse steps (adjoint.py:388-414 inject(1), retreat,
+// then the next step's inject(0)): the H_old terms of step t = step[0] and
+// then every term of step t - 1, added per target cell in that order (the
+// additions of the two launches they replace), then step[0] = t - 1 by the
+// last workgroup to finish (every workgroup read step[0] before its ticket).
+__global__ __launch_bounds__(kBlock) void k_inject_turn(InjectMany a, float* __restrict__ qe,
+                                                        float* __restrict__ qh,
+                                                        int* __restrict__ step,
+                                                        int* __restrict__ ticket, int nx,
+                                                        int ny, int nz) {
+  const int t = step[0];
+  int k, x, y, z;
+  if (inject_target(a, nx, ny, nz, k, x, y, z)) {
+    const int n = nx * ny * nz;
+    const int i = (x * ny + y) * nz + z;
+    float e[3], h[3];
+    for (int c = 0; c < 3; ++c) {
+      e[c] = qe[c * n + i];
+      h[c] = qh[c * n + i];
+    }
+    for (int l = k; l < a.count; ++l)
+      if (l == k || in_targets(a.g[l], x, y, z)) add_plane(a.g[l], t, x, y, z, 1, e, h);
+    for (int l = k; l < a.count; ++l)
+      if (l == k || in_targets(a.g[l], x, y, z)) add_plane(a.g[l], t - 1, x, y, z, 0, e, h);
+    for (int c = 0; c < 3; ++c) {
+      qe[c * n + i] = e[c];
+      qh[c * n + i] = h[c];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && atomicAdd(ticket, 1) == (int)gridDim.x - 1) {
+    step[0] = t - 1;
+    *ticket = 0;
+  }
 }
 
 // adjoint.py:472-490
@@ -497,6 +543,34 @@ int wfd_adjoint_inject_many(const wfd_inject_source* sources, int count, float* 
   hipLaunchKernelGGL(k_inject_many, dim3(blocks), dim3(kBlock), 0, as_stream(stream), a, qe,
                      qh, step, nx, ny, nz, old_h_only);
   WFD_LAUNCH_CHECK("inject_many");
+  return WFD_OK;
+}
+
+int wfd_adjoint_inject_turn(const wfd_inject_source* sources, int count, float* qe, float* qh,
+                            int* step, int* ticket, int nx, int ny, int nz, void* stream) {
+  WFD_REQUIRE((sources || count == 0) && qe && qh && step && ticket,
+              "inject_turn: null pointer");
+  WFD_REQUIRE(count >= 0 && count <= WFD_MAX_INJECT, "inject_turn: 0..%d planes",
+              WFD_MAX_INJECT);
+  InjectMany a;
+  a.count = count;
+  int blocks = 0;
+  for (int k = 0; k < count; ++k) {
+    const wfd_inject_source& d = sources[k];
+    WFD_REQUIRE(d.source && d.weights && d.steps > 0, "inject_turn: plane %d arrays", k);
+    WFD_REQUIRE(d.count > 0 && d.sy > 0 && d.sz > 0 && d.count % (d.sy * d.sz) == 0,
+                "inject_turn: plane %d shape", k);
+    const int sx = d.count / (d.sy * d.sz);
+    if (int s = check_box(nx, ny, nz, d.x0, d.y0, d.z0, sx, d.sy, d.sz)) return s;
+    a.g[k] = InjectSource{d.source, d.weights, d.x0, d.y0, d.z0, d.sy, d.sz, d.count, d.steps};
+    a.begin[k] = blocks;
+    blocks += ((sx + 1) * (d.sy + 1) * (d.sz + 1) + kBlock - 1) / kBlock;
+  }
+  a.begin[count] = blocks;
+  if (blocks == 0) blocks = 1;  // still retreats the step
+  hipLaunchKernelGGL(k_inject_turn, dim3(blocks), dim3(kBlock), 0, as_stream(stream), a, qe,
+                     qh, step, ticket, nx, ny, nz);
+  WFD_LAUNCH_CHECK("inject_turn");
   return WFD_OK;
 }
 

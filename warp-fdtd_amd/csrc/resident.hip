@@ -13,11 +13,18 @@
 // is the one it needs, so a face value and the proof that it is current
 // arrive together: no flags, no release/acquire fences and no cache
 // write-back or invalidation (those cost more than the arithmetic here).
-// Tags within a launch: H faces of the state after k steps = 2k + 1, E faces
-// after the E half of step k = 2k + 2; the words are zeroed before each
-// launch, so a tag never matches a previous launch's word.  A writer can
-// only overwrite a word its reader has consumed: overwriting needs the
-// reader's own next faces, which the reader writes after consuming.
+// Tags: a launch starts from the workspace's epoch word b (read once) and
+// tags the H faces of the state after k of its steps b + 2k + 1 and the E
+// faces after the E half of its step k b + 2k + 2; the counter kernel that
+// follows the launch advances the epoch past every tag of it (by 2n + 2 for
+// n steps).  Every face word is rewritten in every step, so at the start of
+// a launch the words hold tags of the launch before it, all below the new
+// base by at most 2n' + 1 (n' < 2^30 steps): modulo 2^32 none equals a tag
+// the new launch waits for, and the words need no clearing between
+// launches -- only once, when the workspace is first used (zeroed).  A
+// writer can only overwrite a word its reader has consumed: overwriting
+// needs the reader's own next faces, which the reader writes after
+// consuming.
 //
 // Inside the chunk only CPML psi (slab cells), the faces and detector staging
 // touch memory; state is loaded at the start and written back at the end.
@@ -58,6 +65,7 @@ struct ResArgs {
   int sx, sy, sz, gx, gy, gz;
   Word* face;
   int* error;
+  const unsigned* epoch;
   int spin_limit;
 };
 
@@ -164,6 +172,7 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
   if (tid == 0) bail = __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
   if (bail) return;
+  const unsigned base = *a.epoch;
   const int t0 = a.counters[0];
   const float S = a.courant;
   const int rows = a.rows;
@@ -218,24 +227,24 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
   }
   __syncthreads();
 
-  // The initial H faces (tag 1): components (1,2) on x, (0,2) on y, (0,1) on z.
+  // The initial H faces (tag base + 1): components (1,2) on x, (0,2) on y, (0,1) on z.
   for (int t = tid; t < 2 * vy * vz; t += kThreads) {
     const int comp = t >= vy * vz, r = t - comp * vy * vz;
     const int ly = r / vz, lz = r - ly * vz;
     put(myface + comp * fyz + ly * sz + lz,
-        H[(1 + comp) * cells + ((vx - 1) * sy + ly) * sz + lz], 1u);
+        H[(1 + comp) * cells + ((vx - 1) * sy + ly) * sz + lz], base + 1u);
   }
   for (int t = tid; t < 2 * vx * vz; t += kThreads) {
     const int comp = t >= vx * vz, r = t - comp * vx * vz;
     const int lx = r / vz, lz = r - lx * vz;
     put(myface + 2 * fyz + comp * fxz + lx * sz + lz,
-        H[2 * comp * cells + (lx * sy + vy - 1) * sz + lz], 1u);
+        H[2 * comp * cells + (lx * sy + vy - 1) * sz + lz], base + 1u);
   }
   for (int t = tid; t < 2 * vx * vy; t += kThreads) {
     const int comp = t >= vx * vy, r = t - comp * vx * vy;
     const int lx = r / vy, ly = r - lx * vy;
     put(myface + 2 * fyz + 2 * fxz + comp * fxy + lx * sy + ly,
-        H[comp * cells + (lx * sy + ly) * sz + vz - 1], 1u);
+        H[comp * cells + (lx * sy + ly) * sz + vz - 1], base + 1u);
   }
 
   // Columns: thread t < vy*vz owns one (y, z) column per half-step and marches
@@ -313,7 +322,7 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
     const int t = t0 + k;
     // ------------------------------------------------------------ E half
     if (active) {
-      const unsigned tag = 2u * k + 1u, out = 2u * k + 2u;
+      const unsigned tag = base + 2u * k + 1u, out = base + 2u * k + 2u;
       Word* const face = opaque_s(a.face);
       Word* const myface = face + (size_t)wg * per;
       const int ly = opaque_v(eyl), lz = opaque_v(ezl), col = ly * sz + lz;
@@ -446,7 +455,7 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
     if (bail) return;
     // ------------------------------------------------------------ H half
     if (active) {
-      const unsigned tag = 2u * k + 2u, out = 2u * k + 3u;
+      const unsigned tag = base + 2u * k + 2u, out = base + 2u * k + 3u;
       Word* const face = opaque_s(a.face);
       Word* const myface = face + (size_t)wg * per;
       const int ly = opaque_v(hyl), lz = opaque_v(hzl), col = ly * sz + lz;
@@ -609,15 +618,22 @@ __global__ __launch_bounds__(kThreads) void k_resident(ResArgs a, int nsteps) {
   }
 }
 
-__global__ void k_zero_words(Word* w, size_t count) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
-       i += (size_t)gridDim.x * blockDim.x)
-    w[i] = 0;
-}
-
-__global__ void k_advance_counters(int* counters, int nsteps) {
+// After a launch of nsteps: the step counters, the face epoch past every
+// tag of the launch and (when given) the error word copied to `status`.
+__global__ void k_advance_counters(int* counters, unsigned* epoch, int nsteps, const int* error,
+                                   int* status) {
   counters[0] += nsteps;
   counters[1] = counters[0] - 1;
+  epoch[0] += 2u * (unsigned)nsteps + 2u;
+  if (status) status[0] = error[0];
+}
+
+// Zero `count` 32-bit words (a kernel rather than hipMemsetAsync, so that the
+// clearing is an ordinary kernel node when a caller captures the stream).
+__global__ void k_fill_zero(unsigned* w, size_t count) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (size_t)gridDim.x * blockDim.x)
+    w[i] = 0u;
 }
 
 struct Layout {
@@ -714,8 +730,16 @@ int wfd_resident_layout_for(const wfd_yee_plan* p, int sx, int sy, int sz,
   return WFD_OK;
 }
 
-int wfd_resident_run(const wfd_yee_plan* p, const wfd_resident_layout* L, void* workspace,
-                     int nsteps, int spin_limit, void* stream) {
+}  // extern "C"
+
+namespace wfd {
+namespace {
+
+// Validate and launch one resident chunk with its face words, error word and
+// epoch at `workspace` (`clear`: zeroed first, for a workspace in an unknown
+// state).  `status` (optional) gets the error word after the launch.
+int resident_launch(const wfd_yee_plan* p, const wfd_resident_layout* L, void* workspace,
+                    bool clear, int nsteps, int spin_limit, int* status, void* stream) {
   WFD_REQUIRE(p && L && workspace, "resident_run: null argument");
   WFD_REQUIRE(nsteps >= 0 && nsteps < (1 << 30), "resident_run: step count out of range");
   WFD_REQUIRE(p->e && p->h && p->inv_eps && p->psi_e && p->psi_h && p->coeff_e && p->coeff_h &&
@@ -777,6 +801,8 @@ int wfd_resident_run(const wfd_yee_plan* p, const wfd_resident_layout* L, void* 
   const size_t words = face_words(L->sx, L->sy, L->sz) * L->blocks;
   a.face = reinterpret_cast<Word*>(workspace);
   a.error = reinterpret_cast<int*>(a.face + words);
+  unsigned* epoch = reinterpret_cast<unsigned*>(a.error + 1);
+  a.epoch = epoch;
   a.spin_limit = spin_limit > 0 ? spin_limit : (1 << 22);
   a.nstage = p->nstage;
   a.slots = p->slots > 0 ? p->slots : 1;
@@ -785,14 +811,29 @@ int wfd_resident_run(const wfd_yee_plan* p, const wfd_resident_layout* L, void* 
     WFD_REQUIRE(to_stage_box(p->stages[b], p->nx, p->ny, p->nz, a.stages[b]) == WFD_OK,
                 "resident_run: stage %d outside the grid or without a buffer", b);
   hipStream_t s = as_stream(stream);
-  hipLaunchKernelGGL(k_zero_words, dim3(256), dim3(256), 0, s, a.face, words);
-  WFD_LAUNCH_CHECK("resident_run(zero faces)");
+  if (clear) {
+    hipLaunchKernelGGL(k_fill_zero, dim3(256), dim3(256), 0, s,
+                       reinterpret_cast<unsigned*>(workspace),
+                       (size_t)(L->workspace_bytes / 4));
+    WFD_LAUNCH_CHECK("resident_run(clear scratch)");
+  }
   void* args[] = {&a, &nsteps};
   WFD_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_resident),
                                      dim3(L->blocks), dim3(kThreads), args, 0, s));
-  hipLaunchKernelGGL(k_advance_counters, dim3(1), dim3(1), 0, s, p->counters, nsteps);
+  hipLaunchKernelGGL(k_advance_counters, dim3(1), dim3(1), 0, s, p->counters, epoch, nsteps,
+                     a.error, status);
   WFD_LAUNCH_CHECK("resident_run(counters)");
   return WFD_OK;
+}
+
+}  // namespace
+}  // namespace wfd
+
+extern "C" {
+
+int wfd_resident_run(const wfd_yee_plan* p, const wfd_resident_layout* L, void* workspace,
+                     int nsteps, int spin_limit, void* stream) {
+  return resident_launch(p, L, workspace, false, nsteps, spin_limit, nullptr, stream);
 }
 
 }  // extern "C"

@@ -484,6 +484,653 @@ __global__ __launch_bounds__(kBlock) void k_batch_h(const YeeArgs* __restrict__ 
   yee_h<XC>(table[d], blockIdx.x - first[d]);
 }
 
+// ------------------------------------------------------------------------
+// Single-pass step: E and H of one step in one launch, reading the step's
+// start (E, H, psi) from one buffer set and writing its result to another
+// (the caller alternates the two), so that a workgroup can recompute the E
+// values its H update needs from the neighbouring tile's cells while that
+// tile writes its own results elsewhere.
+//
+// A workgroup owns a tile of W consecutive (y, z) offsets o in [A, A + W)
+// (o = y*nz + z) over a chunk of x planes [x0, x1) and marches x = x0 .. x1:
+// at plane x it computes the new E over [A, A + W + nz) -- the nz cells past
+// the tile are the y + 1 neighbours of its last row, recomputed and not
+// stored -- and puts it in LDS; then the new H of plane x - 1 over the tile
+// from the new E of plane x - 1 (LDS: the y + 1 and z + 1 neighbours) and of
+// plane x (registers: the x + 1 neighbour).  Plane x1, the next chunk's first,
+// is recomputed the same way.  Each cell's E, H and 1/eps are read once and
+// its E and H written once (52 B per cell-update plus CPML psi); the halo
+// rows and planes are re-read by the neighbouring workgroups through L2.
+// Thread t of the block handles the offsets A + t + kBlock * j, j < J.
+struct PassArgs {
+  YeeArgs a;     // the step's start (e, h, psi_e, psi_h) and every table
+  float* e_out;  // the step's result
+  float* h_out;
+  float* pe_out;
+  float* ph_out;
+  int* ticket;   // workgroups finished (the last one advances the counters)
+  int W;         // tile cells (W + nz <= J * kBlock)
+  int tiles;     // tiles per plane
+  int xc;        // x planes per chunk
+  int total;     // tiles * chunks
+  int per;       // logical workgroups per XCD stripe
+};
+
+template <int J>
+__global__ __launch_bounds__(kBlock) void k_yee_pass(PassArgs pa) {
+  constexpr int SLOTS = J * kBlock;
+  __shared__ float lds[2][3][SLOTS];  // the new E of planes x - 1 and x
+  const YeeArgs& a = pa.a;
+  const int logical = (blockIdx.x & 7) * pa.per + (blockIdx.x >> 3);
+  const int t = a.counters[0];
+  if (logical < pa.total) {
+    const int nx = a.nx, ny = a.ny, nz = a.nz, n = a.n, P = a.P, rows = a.rows;
+    const int tile = logical % pa.tiles, chunk = logical / pa.tiles;
+    const int A = tile * pa.W;
+    const int x0 = chunk * pa.xc, x1 = min(x0 + pa.xc, nx);
+    const int B = min(A + pa.W, P), Bx = min(A + pa.W + nz, P);
+    const float* __restrict__ ce = a.coeff_e;
+    const float* __restrict__ ch = a.coeff_h;
+    const float* __restrict__ E0 = a.e;
+    const float* __restrict__ H0 = a.h;
+    const float* __restrict__ pe0 = a.psi_e;
+    const float* __restrict__ ph0 = a.psi_h;
+    float* __restrict__ E1 = pa.e_out;
+    float* __restrict__ H1 = pa.h_out;
+    float* __restrict__ pe1 = pa.pe_out;
+    float* __restrict__ ph1 = pa.ph_out;
+    const float S = a.courant;
+    const bool src_on = t < a.steps && a.source_x >= x0 && a.source_x <= x1;
+    const float wave_hy = src_on ? a.wave_h[2 * a.steps + t] : 0.f;
+    const float wave_hz = src_on ? a.wave_h[a.steps + t] : 0.f;
+    const float wave_ey = src_on ? a.wave_e[2 * a.steps + t] : 0.f;
+    const float wave_ez = src_on ? a.wave_e[a.steps + t] : 0.f;
+    const bool save_t = t >= a.save_h_from;
+    // per slot: offset, position and the values carried along x
+    int o[J], y[J], z[J];
+    float hp[J][3];  // H at the plane before (the step's start)
+    float ep[J][3];  // new E at the plane before
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      o[j] = A + threadIdx.x + kBlock * j;
+      const int oc = o[j] < Bx ? o[j] : A;
+      y[j] = div_small(oc, nz, a.inv_nz);
+      z[j] = oc - y[j] * nz;
+      hp[j][0] = hp[j][1] = hp[j][2] = 0.f;
+      ep[j][0] = ep[j][1] = ep[j][2] = 0.f;
+      if (o[j] < Bx && x0 > 0) {
+        const int i = (x0 - 1) * P + o[j];
+        hp[j][1] = H0[n + i];
+        hp[j][2] = H0[2 * n + i];
+      }
+    }
+#pragma unroll 1
+    for (int x = x0; x <= x1; ++x) {
+      float* __restrict__ cur = &lds[x & 1][0][0];
+      const float* __restrict__ prev = &lds[(x + 1) & 1][0][0];
+      // the new E of plane x over the tile and its y + 1 rows
+      float en[J][3], hn[J][3];
+      if (x < nx) {
+        const float xa = ce[x * 3], xb = ce[x * 3 + 1], xk = ce[x * 3 + 2];
+        // loads first (every slot), then the arithmetic
+        float hx[J], yx[J], yz[J], zx[J], zy[J], ex[J], ey[J], ez[J], ie[J], ps[J][6];
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+          hn[j][0] = hn[j][1] = hn[j][2] = 0.f;
+          if (o[j] < Bx) {
+            const int i = x * P + o[j];
+            hn[j][0] = H0[i];
+            hn[j][1] = H0[n + i];
+            hn[j][2] = H0[2 * n + i];
+            yx[j] = y[j] > 0 ? H0[i - nz] : 0.f;
+            yz[j] = y[j] > 0 ? H0[2 * n + i - nz] : 0.f;
+            zx[j] = z[j] > 0 ? H0[i - 1] : 0.f;
+            zy[j] = z[j] > 0 ? H0[n + i - 1] : 0.f;
+            ex[j] = E0[i];
+            ey[j] = E0[n + i];
+            ez[j] = E0[2 * n + i];
+            ie[j] = a.inv_eps[i];
+            const float ya = ce[(rows + y[j]) * 3], za = ce[(2 * rows + z[j]) * 3];
+            ps[j][0] = xa != 0.f ? pe0[i] : 0.f;
+            ps[j][1] = xa != 0.f ? pe0[n + i] : 0.f;
+            ps[j][2] = ya != 0.f ? pe0[2 * n + i] : 0.f;
+            ps[j][3] = ya != 0.f ? pe0[3 * n + i] : 0.f;
+            ps[j][4] = za != 0.f ? pe0[4 * n + i] : 0.f;
+            ps[j][5] = za != 0.f ? pe0[5 * n + i] : 0.f;
+          }
+          hx[j] = hn[j][0];
+        }
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+          en[j][0] = en[j][1] = en[j][2] = 0.f;
+          if (o[j] < Bx) {
+            const int i = x * P + o[j];
+            const bool keep = x < x1 && o[j] < B;  // owned: stored
+            const float ya = ce[(rows + y[j]) * 3], yb = ce[(rows + y[j]) * 3 + 1],
+                        yk = ce[(rows + y[j]) * 3 + 2];
+            const float za = ce[(2 * rows + z[j]) * 3], zb = ce[(2 * rows + z[j]) * 3 + 1],
+                        zk = ce[(2 * rows + z[j]) * 3 + 2];
+            const float dx1 = hn[j][1] - hp[j][1], dx2 = hn[j][2] - hp[j][2];
+            const float dy0 = hx[j] - yx[j], dy2 = hn[j][2] - yz[j];
+            const float dz0 = hx[j] - zx[j], dz1 = hn[j][1] - zy[j];
+            float dxz, dxy, dyx, dyz, dzy, dzx;
+            if (xa != 0.f) {
+              const float p0 = xb * ps[j][0] + xa * dx2;
+              const float p1 = xb * ps[j][1] + xa * dx1;
+              if (keep) {
+                pe1[i] = p0;
+                pe1[n + i] = p1;
+              }
+              dxz = xk * dx2 + p0;
+              dxy = xk * dx1 + p1;
+            } else {
+              dxz = xk * dx2;
+              dxy = xk * dx1;
+            }
+            if (ya != 0.f) {
+              const float p2 = yb * ps[j][2] + ya * dy0;
+              const float p3 = yb * ps[j][3] + ya * dy2;
+              if (keep) {
+                pe1[2 * n + i] = p2;
+                pe1[3 * n + i] = p3;
+              }
+              dyx = yk * dy0 + p2;
+              dyz = yk * dy2 + p3;
+            } else {
+              dyx = yk * dy0;
+              dyz = yk * dy2;
+            }
+            if (za != 0.f) {
+              const float p4 = zb * ps[j][4] + za * dz1;
+              const float p5 = zb * ps[j][5] + za * dz0;
+              if (keep) {
+                pe1[4 * n + i] = p4;
+                pe1[5 * n + i] = p5;
+              }
+              dzy = zk * dz1 + p4;
+              dzx = zk * dz0 + p5;
+            } else {
+              dzy = zk * dz1;
+              dzx = zk * dz0;
+            }
+            float vx = ex[j] + (S * (dyz - dzy)) * ie[j];
+            float vy = ey[j] + (S * (dzx - dxz)) * ie[j];
+            float vz = ez[j] + (S * (dxy - dyx)) * ie[j];
+            if (src_on && x == a.source_x) {
+              vy += ((a.src_h[2 * P + o[j]] * wave_hy) * S) * ie[j];
+              vz -= ((a.src_h[P + o[j]] * wave_hz) * S) * ie[j];
+            }
+            en[j][0] = vx;
+            en[j][1] = vy;
+            en[j][2] = vz;
+            if (keep) {
+              E1[i] = vx;
+              E1[n + i] = vy;
+              E1[2 * n + i] = vz;
+            }
+          }
+          const int s = threadIdx.x + kBlock * j;
+          cur[s] = en[j][0];
+          cur[SLOTS + s] = en[j][1];
+          cur[2 * SLOTS + s] = en[j][2];
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < J; ++j)
+          en[j][0] = en[j][1] = en[j][2] = hn[j][0] = hn[j][1] = hn[j][2] = 0.f;
+      }
+      // the new H of plane x - 1 over the tile
+      if (x > x0) {
+        const int xh = x - 1;
+        const float xa = ch[xh * 3], xb = ch[xh * 3 + 1], xk = ch[xh * 3 + 2];
+        float ps[J][6];
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+          if (o[j] < B) {
+            const int i = xh * P + o[j];
+            const float ya = ch[(rows + y[j]) * 3], za = ch[(2 * rows + z[j]) * 3];
+            ps[j][0] = xa != 0.f ? ph0[i] : 0.f;
+            ps[j][1] = xa != 0.f ? ph0[n + i] : 0.f;
+            ps[j][2] = ya != 0.f ? ph0[2 * n + i] : 0.f;
+            ps[j][3] = ya != 0.f ? ph0[3 * n + i] : 0.f;
+            ps[j][4] = za != 0.f ? ph0[4 * n + i] : 0.f;
+            ps[j][5] = za != 0.f ? ph0[5 * n + i] : 0.f;
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+          if (o[j] < B) {
+            const int i = xh * P + o[j];
+            const int s = threadIdx.x + kBlock * j;
+            const bool up_y = y[j] + 1 < ny, up_z = z[j] + 1 < nz;
+            const float qx = up_y ? prev[s + nz] : 0.f;
+            const float qz = up_y ? prev[2 * SLOTS + s + nz] : 0.f;
+            const float rx = up_z ? prev[s + 1] : 0.f;
+            const float ry = up_z ? prev[SLOTS + s + 1] : 0.f;
+            const float dx1 = en[j][1] - ep[j][1], dx2 = en[j][2] - ep[j][2];
+            const float dy0 = qx - ep[j][0], dy2 = qz - ep[j][2];
+            const float dz0 = rx - ep[j][0], dz1 = ry - ep[j][1];
+            const float ya = ch[(rows + y[j]) * 3], yb = ch[(rows + y[j]) * 3 + 1],
+                        yk = ch[(rows + y[j]) * 3 + 2];
+            const float za = ch[(2 * rows + z[j]) * 3], zb = ch[(2 * rows + z[j]) * 3 + 1],
+                        zk = ch[(2 * rows + z[j]) * 3 + 2];
+            float dxz, dxy, dyx, dyz, dzy, dzx;
+            if (xa != 0.f) {
+              const float p0 = xb * ps[j][0] + xa * dx2;
+              const float p1 = xb * ps[j][1] + xa * dx1;
+              ph1[i] = p0;
+              ph1[n + i] = p1;
+              dxz = xk * dx2 + p0;
+              dxy = xk * dx1 + p1;
+            } else {
+              dxz = xk * dx2;
+              dxy = xk * dx1;
+            }
+            if (ya != 0.f) {
+              const float p2 = yb * ps[j][2] + ya * dy0;
+              const float p3 = yb * ps[j][3] + ya * dy2;
+              ph1[2 * n + i] = p2;
+              ph1[3 * n + i] = p3;
+              dyx = yk * dy0 + p2;
+              dyz = yk * dy2 + p3;
+            } else {
+              dyx = yk * dy0;
+              dyz = yk * dy2;
+            }
+            if (za != 0.f) {
+              const float p4 = zb * ps[j][4] + za * dz1;
+              const float p5 = zb * ps[j][5] + za * dz0;
+              ph1[4 * n + i] = p4;
+              ph1[5 * n + i] = p5;
+              dzy = zk * dz1 + p4;
+              dzx = zk * dz0 + p5;
+            } else {
+              dzy = zk * dz1;
+              dzx = zk * dz0;
+            }
+            const float hx = hp[j][0], hy = hp[j][1], hz = hp[j][2];
+            if (save_t && xh >= a.hbox[0] && xh < a.hbox[1] && y[j] >= a.hbox[2] &&
+                y[j] < a.hbox[3] && z[j] >= a.hbox[4] && z[j] < a.hbox[5]) {
+              a.hprev[i] = hx;
+              a.hprev[n + i] = hy;
+              a.hprev[2 * n + i] = hz;
+            }
+            float vx = hx - S * (dyz - dzy);
+            float vy = hy - S * (dzx - dxz);
+            float vz = hz - S * (dxy - dyx);
+            if (src_on && xh == a.source_x) {
+              vy -= (a.src_e[2 * P + o[j]] * wave_ey) * S;
+              vz += (a.src_e[P + o[j]] * wave_ez) * S;
+            }
+            H1[i] = vx;
+            H1[n + i] = vy;
+            H1[2 * n + i] = vz;
+            if (a.nstage)
+              stage_cell(a.st, a.nstage, t % a.slots, xh, y[j], z[j], ep[j][0], ep[j][1],
+                         ep[j][2], vx, vy, vz, hx, hy, hz);
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < J; ++j)
+        for (int c = 0; c < 3; ++c) {
+          hp[j][c] = hn[j][c];
+          ep[j][c] = en[j][c];
+        }
+      __syncthreads();
+    }
+  }
+  // the last workgroup to finish advances the step counters (every workgroup
+  // read counters[0] before it took its ticket)
+  if (threadIdx.x == 0) {
+    const int done = atomicAdd(pa.ticket, 1);
+    if (done == (int)gridDim.x - 1) {
+      a.counters[0] = t + 1;
+      a.counters[1] = t;
+      *pa.ticket = 0;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// Two steps per pass (temporal blocking): the step above done twice in one
+// march, the first step's E and H (E1, H1) and CPML psi never leaving the
+// chip.  Per (y, z) offset, in flattened o = y*nz + z, a tile [A, B) of the
+// second step's H needs the second step's E on [A, B + nz), the first step's
+// H on [A - nz, B + nz) and its E on [A - nz, B + 2 nz); the slots of a
+// workgroup cover [A - nz, B + 2 nz).  Along x, the march at plane x computes
+// E1(x), H1(x - 1), E2(x - 1) and H2(x - 2); the chunk [x0, x1) of outputs
+// needs E1 on [x0 - 1, x1 + 1].  Planes of E1 and E2 go through LDS (double
+// buffered: the y + 1 and z + 1 neighbours of the plane before), the plane of
+// H1 too (the y - 1 and z - 1 neighbours of its own plane); every value
+// needed along x stays in the registers of the thread that holds its
+// offset.  Reads E, H, 1/eps once and writes E, H once per two
+// cell-updates; psi is read and written once per two steps as well.
+struct Rows {
+  float xa, xb, xk, ya, yb, yk, za, zb, zk;
+};
+
+__device__ __forceinline__ Rows rows_at(const float* __restrict__ c, int rows, int x, int y,
+                                        int z) {
+  Rows r;
+  r.xa = c[x * 3];
+  r.xb = c[x * 3 + 1];
+  r.xk = c[x * 3 + 2];
+  r.ya = c[(rows + y) * 3];
+  r.yb = c[(rows + y) * 3 + 1];
+  r.yk = c[(rows + y) * 3 + 2];
+  r.za = c[(2 * rows + z) * 3];
+  r.zb = c[(2 * rows + z) * 3 + 1];
+  r.zk = c[(2 * rows + z) * 3 + 2];
+  return r;
+}
+
+// The six CPML partials of one cell (kernels.py:29-47, the fused step's
+// operations); ps[] becomes the updated psi where the axis has CPML.
+__device__ __forceinline__ void partials(const Rows& r, float dx1, float dx2, float dy0,
+                                         float dy2, float dz0, float dz1, float (&ps)[6],
+                                         float& dxz, float& dxy, float& dyx, float& dyz,
+                                         float& dzy, float& dzx) {
+  if (r.xa != 0.f) {
+    const float p0 = r.xb * ps[0] + r.xa * dx2;
+    const float p1 = r.xb * ps[1] + r.xa * dx1;
+    ps[0] = p0;
+    ps[1] = p1;
+    dxz = r.xk * dx2 + p0;
+    dxy = r.xk * dx1 + p1;
+  } else {
+    dxz = r.xk * dx2;
+    dxy = r.xk * dx1;
+  }
+  if (r.ya != 0.f) {
+    const float p2 = r.yb * ps[2] + r.ya * dy0;
+    const float p3 = r.yb * ps[3] + r.ya * dy2;
+    ps[2] = p2;
+    ps[3] = p3;
+    dyx = r.yk * dy0 + p2;
+    dyz = r.yk * dy2 + p3;
+  } else {
+    dyx = r.yk * dy0;
+    dyz = r.yk * dy2;
+  }
+  if (r.za != 0.f) {
+    const float p4 = r.zb * ps[4] + r.za * dz1;
+    const float p5 = r.zb * ps[5] + r.za * dz0;
+    ps[4] = p4;
+    ps[5] = p5;
+    dzy = r.zk * dz1 + p4;
+    dzx = r.zk * dz0 + p5;
+  } else {
+    dzy = r.zk * dz1;
+    dzx = r.zk * dz0;
+  }
+}
+
+__device__ __forceinline__ void load_psi6(const float* __restrict__ psi, int n, int i,
+                                          const Rows& r, float (&ps)[6]) {
+  ps[0] = r.xa != 0.f ? psi[i] : 0.f;
+  ps[1] = r.xa != 0.f ? psi[n + i] : 0.f;
+  ps[2] = r.ya != 0.f ? psi[2 * n + i] : 0.f;
+  ps[3] = r.ya != 0.f ? psi[3 * n + i] : 0.f;
+  ps[4] = r.za != 0.f ? psi[4 * n + i] : 0.f;
+  ps[5] = r.za != 0.f ? psi[5 * n + i] : 0.f;
+}
+
+__device__ __forceinline__ void store_psi6(float* __restrict__ psi, int n, int i, const Rows& r,
+                                           const float (&ps)[6]) {
+  if (r.xa != 0.f) {
+    psi[i] = ps[0];
+    psi[n + i] = ps[1];
+  }
+  if (r.ya != 0.f) {
+    psi[2 * n + i] = ps[2];
+    psi[3 * n + i] = ps[3];
+  }
+  if (r.za != 0.f) {
+    psi[4 * n + i] = ps[4];
+    psi[5 * n + i] = ps[5];
+  }
+}
+
+template <int T, int J>
+__global__ __launch_bounds__(T) void k_yee_pass2(PassArgs pa) {
+  constexpr int SLOTS = J * T;
+  __shared__ float le1[2][3][SLOTS];  // E1 of planes x - 1 and x
+  __shared__ float lh1[3][SLOTS];     // H1 of plane x - 1
+  __shared__ float le2[2][3][SLOTS];  // E2 of planes x - 2 and x - 1
+  const YeeArgs& a = pa.a;
+  const int logical = (blockIdx.x & 7) * pa.per + (blockIdx.x >> 3);
+  const int t = a.counters[0];
+  if (logical < pa.total) {
+    const int nx = a.nx, ny = a.ny, nz = a.nz, n = a.n, P = a.P, rows = a.rows;
+    const int tile = logical % pa.tiles, chunk = logical / pa.tiles;
+    const int A = tile * pa.W;
+    const int x0 = chunk * pa.xc, x1 = min(x0 + pa.xc, nx);
+    const int B = min(A + pa.W, P);
+    const int Bh = min(B + nz, P), Be = min(B + 2 * nz, P);  // H1 and E1 range ends
+    const int lo = A - nz;                                   // offset of slot 0
+    const float* __restrict__ ce = a.coeff_e;
+    const float* __restrict__ ch = a.coeff_h;
+    const float* __restrict__ E0 = a.e;
+    const float* __restrict__ H0 = a.h;
+    const float S = a.courant;
+    // sources of the two steps
+    const bool src1 = t < a.steps && a.source_x >= x0 - 1 && a.source_x <= x1 + 1;
+    const bool src2 = t + 1 < a.steps && a.source_x >= x0 - 1 && a.source_x <= x1 + 1;
+    const float wh1y = src1 ? a.wave_h[2 * a.steps + t] : 0.f;
+    const float wh1z = src1 ? a.wave_h[a.steps + t] : 0.f;
+    const float we1y = src1 ? a.wave_e[2 * a.steps + t] : 0.f;
+    const float we1z = src1 ? a.wave_e[a.steps + t] : 0.f;
+    const float wh2y = src2 ? a.wave_h[2 * a.steps + t + 1] : 0.f;
+    const float wh2z = src2 ? a.wave_h[a.steps + t + 1] : 0.f;
+    const float we2y = src2 ? a.wave_e[2 * a.steps + t + 1] : 0.f;
+    const float we2z = src2 ? a.wave_e[a.steps + t + 1] : 0.f;
+    const bool save = t + 1 >= a.save_h_from;
+    const int xs = max(x0 - 1, 0);
+    int o[J], y[J], z[J];
+    float hp0[J][3], e1p[J][3], pse1[J][6], ie1[J], h1p[J][3], psh1[J][6], e2p[J][3];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      o[j] = lo + (int)threadIdx.x + T * j;
+      const int oc = (o[j] >= 0 && o[j] < Be) ? o[j] : max(A, 0);
+      y[j] = div_small(oc, nz, a.inv_nz);
+      z[j] = oc - y[j] * nz;
+      for (int c = 0; c < 3; ++c) hp0[j][c] = e1p[j][c] = h1p[j][c] = e2p[j][c] = 0.f;
+      for (int c = 0; c < 6; ++c) pse1[j][c] = psh1[j][c] = 0.f;
+      ie1[j] = 0.f;
+      if (o[j] >= 0 && o[j] < Be && xs > 0) {
+        const int i = (xs - 1) * P + o[j];
+        hp0[j][1] = H0[n + i];
+        hp0[j][2] = H0[2 * n + i];
+      }
+    }
+#pragma unroll 1
+    for (int x = xs; x <= x1 + 1; ++x) {
+      float en[J][3], hn[J][3], pse[J][6], iex[J], h1n[J][3], psh[J][6], e2n[J][3];
+      // (a) E1 of plane x over [A - nz, B + 2 nz)
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        for (int c = 0; c < 3; ++c) en[j][c] = hn[j][c] = 0.f;
+        for (int c = 0; c < 6; ++c) pse[j][c] = 0.f;
+        iex[j] = 0.f;
+        if (x < nx && o[j] >= 0 && o[j] < Be) {
+          const int i = x * P + o[j];
+          const Rows r = rows_at(ce, rows, x, y[j], z[j]);
+          hn[j][0] = H0[i];
+          hn[j][1] = H0[n + i];
+          hn[j][2] = H0[2 * n + i];
+          const float yx = y[j] > 0 ? H0[i - nz] : 0.f;
+          const float yz = y[j] > 0 ? H0[2 * n + i - nz] : 0.f;
+          const float zx = z[j] > 0 ? H0[i - 1] : 0.f;
+          const float zy = z[j] > 0 ? H0[n + i - 1] : 0.f;
+          const float ex = E0[i], ey = E0[n + i], ez = E0[2 * n + i];
+          iex[j] = a.inv_eps[i];
+          load_psi6(a.psi_e, n, i, r, pse[j]);
+          float dxz, dxy, dyx, dyz, dzy, dzx;
+          partials(r, hn[j][1] - hp0[j][1], hn[j][2] - hp0[j][2], hn[j][0] - yx, hn[j][2] - yz,
+                   hn[j][0] - zx, hn[j][1] - zy, pse[j], dxz, dxy, dyx, dyz, dzy, dzx);
+          float vx = ex + (S * (dyz - dzy)) * iex[j];
+          float vy = ey + (S * (dzx - dxz)) * iex[j];
+          float vz = ez + (S * (dxy - dyx)) * iex[j];
+          if (src1 && x == a.source_x) {
+            vy += ((a.src_h[2 * P + o[j]] * wh1y) * S) * iex[j];
+            vz -= ((a.src_h[P + o[j]] * wh1z) * S) * iex[j];
+          }
+          en[j][0] = vx;
+          en[j][1] = vy;
+          en[j][2] = vz;
+        }
+        const int s = threadIdx.x + T * j;
+        le1[x & 1][0][s] = en[j][0];
+        le1[x & 1][1][s] = en[j][1];
+        le1[x & 1][2][s] = en[j][2];
+      }
+      // (b) H1 of plane x - 1 over [A - nz, B + nz)
+      const int xh = x - 1;
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        for (int c = 0; c < 3; ++c) h1n[j][c] = 0.f;
+        for (int c = 0; c < 6; ++c) psh[j][c] = 0.f;
+        if (xh >= x0 - 1 && xh >= 0 && xh < nx && o[j] >= 0 && o[j] < Bh) {
+          const int i = xh * P + o[j];
+          const int s = threadIdx.x + T * j;
+          const Rows r = rows_at(ch, rows, xh, y[j], z[j]);
+          load_psi6(a.psi_h, n, i, r, psh[j]);
+          const bool up_y = y[j] + 1 < ny, up_z = z[j] + 1 < nz;
+          const float(*prev)[SLOTS] = le1[xh & 1];
+          const float qx = up_y ? prev[0][s + nz] : 0.f;
+          const float qz = up_y ? prev[2][s + nz] : 0.f;
+          const float rx = up_z ? prev[0][s + 1] : 0.f;
+          const float ry = up_z ? prev[1][s + 1] : 0.f;
+          float dxz, dxy, dyx, dyz, dzy, dzx;
+          partials(r, en[j][1] - e1p[j][1], en[j][2] - e1p[j][2], qx - e1p[j][0],
+                   qz - e1p[j][2], rx - e1p[j][0], ry - e1p[j][1], psh[j], dxz, dxy, dyx, dyz,
+                   dzy, dzx);
+          float vx = hp0[j][0] - S * (dyz - dzy);
+          float vy = hp0[j][1] - S * (dzx - dxz);
+          float vz = hp0[j][2] - S * (dxy - dyx);
+          if (src1 && xh == a.source_x) {
+            vy -= (a.src_e[2 * P + o[j]] * we1y) * S;
+            vz += (a.src_e[P + o[j]] * we1z) * S;
+          }
+          h1n[j][0] = vx;
+          h1n[j][1] = vy;
+          h1n[j][2] = vz;
+          if (a.nstage && o[j] >= A && o[j] < B && xh >= x0 && xh < x1)
+            stage_cell(a.st, a.nstage, t % a.slots, xh, y[j], z[j], e1p[j][0], e1p[j][1],
+                       e1p[j][2], vx, vy, vz, hp0[j][0], hp0[j][1], hp0[j][2]);
+        }
+        const int s = threadIdx.x + T * j;
+        lh1[0][s] = h1n[j][0];
+        lh1[1][s] = h1n[j][1];
+        lh1[2][s] = h1n[j][2];
+      }
+      __syncthreads();
+      // (c) E2 of plane x - 1 over [A, B + nz)
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        for (int c = 0; c < 3; ++c) e2n[j][c] = 0.f;
+        if (xh >= x0 && xh <= x1 && xh < nx && o[j] >= A && o[j] < Bh) {
+          const int i = xh * P + o[j];
+          const int s = threadIdx.x + T * j;
+          const Rows r = rows_at(ce, rows, xh, y[j], z[j]);
+          const float yx = y[j] > 0 ? lh1[0][s - nz] : 0.f;
+          const float yz = y[j] > 0 ? lh1[2][s - nz] : 0.f;
+          const float zx = z[j] > 0 ? lh1[0][s - 1] : 0.f;
+          const float zy = z[j] > 0 ? lh1[1][s - 1] : 0.f;
+          float dxz, dxy, dyx, dyz, dzy, dzx;
+          partials(r, h1n[j][1] - h1p[j][1], h1n[j][2] - h1p[j][2], h1n[j][0] - yx,
+                   h1n[j][2] - yz, h1n[j][0] - zx, h1n[j][1] - zy, pse1[j], dxz, dxy, dyx, dyz,
+                   dzy, dzx);
+          float vx = e1p[j][0] + (S * (dyz - dzy)) * ie1[j];
+          float vy = e1p[j][1] + (S * (dzx - dxz)) * ie1[j];
+          float vz = e1p[j][2] + (S * (dxy - dyx)) * ie1[j];
+          if (src2 && xh == a.source_x) {
+            vy += ((a.src_h[2 * P + o[j]] * wh2y) * S) * ie1[j];
+            vz -= ((a.src_h[P + o[j]] * wh2z) * S) * ie1[j];
+          }
+          e2n[j][0] = vx;
+          e2n[j][1] = vy;
+          e2n[j][2] = vz;
+          if (o[j] < B && xh < x1) {
+            pa.e_out[i] = vx;
+            pa.e_out[n + i] = vy;
+            pa.e_out[2 * n + i] = vz;
+            store_psi6(pa.pe_out, n, i, r, pse1[j]);
+          }
+        }
+        const int s = threadIdx.x + T * j;
+        le2[xh & 1][0][s] = e2n[j][0];
+        le2[xh & 1][1][s] = e2n[j][1];
+        le2[xh & 1][2][s] = e2n[j][2];
+      }
+      // (d) H2 of plane x - 2 over the tile
+      const int xq = x - 2;
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        if (xq >= x0 && xq < x1 && o[j] >= A && o[j] < B) {
+          const int i = xq * P + o[j];
+          const int s = threadIdx.x + T * j;
+          const Rows r = rows_at(ch, rows, xq, y[j], z[j]);
+          const bool up_y = y[j] + 1 < ny, up_z = z[j] + 1 < nz;
+          const float(*prev)[SLOTS] = le2[xq & 1];
+          const float qx = up_y ? prev[0][s + nz] : 0.f;
+          const float qz = up_y ? prev[2][s + nz] : 0.f;
+          const float rx = up_z ? prev[0][s + 1] : 0.f;
+          const float ry = up_z ? prev[1][s + 1] : 0.f;
+          float dxz, dxy, dyx, dyz, dzy, dzx;
+          partials(r, e2n[j][1] - e2p[j][1], e2n[j][2] - e2p[j][2], qx - e2p[j][0],
+                   qz - e2p[j][2], rx - e2p[j][0], ry - e2p[j][1], psh1[j], dxz, dxy, dyx, dyz,
+                   dzy, dzx);
+          float vx = h1p[j][0] - S * (dyz - dzy);
+          float vy = h1p[j][1] - S * (dzx - dxz);
+          float vz = h1p[j][2] - S * (dxy - dyx);
+          if (src2 && xq == a.source_x) {
+            vy -= (a.src_e[2 * P + o[j]] * we2y) * S;
+            vz += (a.src_e[P + o[j]] * we2z) * S;
+          }
+          if (save && xq >= a.hbox[0] && xq < a.hbox[1] && y[j] >= a.hbox[2] &&
+              y[j] < a.hbox[3] && z[j] >= a.hbox[4] && z[j] < a.hbox[5]) {
+            a.hprev[i] = h1p[j][0];
+            a.hprev[n + i] = h1p[j][1];
+            a.hprev[2 * n + i] = h1p[j][2];
+          }
+          pa.h_out[i] = vx;
+          pa.h_out[n + i] = vy;
+          pa.h_out[2 * n + i] = vz;
+          store_psi6(pa.ph_out, n, i, r, psh1[j]);
+          if (a.nstage)
+            stage_cell(a.st, a.nstage, (t + 1) % a.slots, xq, y[j], z[j], e2p[j][0], e2p[j][1],
+                       e2p[j][2], vx, vy, vz, h1p[j][0], h1p[j][1], h1p[j][2]);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        for (int c = 0; c < 3; ++c) {
+          hp0[j][c] = hn[j][c];
+          e1p[j][c] = en[j][c];
+          h1p[j][c] = h1n[j][c];
+          e2p[j][c] = e2n[j][c];
+        }
+        for (int c = 0; c < 6; ++c) {
+          pse1[j][c] = pse[j][c];
+          psh1[j][c] = psh[j][c];
+        }
+        ie1[j] = iex[j];
+      }
+      __syncthreads();
+    }
+  }
+  if (threadIdx.x == 0) {
+    const int done = atomicAdd(pa.ticket, 1);
+    if (done == (int)gridDim.x - 1) {
+      a.counters[0] = t + 2;
+      a.counters[1] = t + 1;
+      *pa.ticket = 0;
+    }
+  }
+}
+
 // Validate a plan and fill the fused step's arguments (shared by the single
 // and the batched step).
 int build_args(const wfd_yee_plan* p, YeeArgs& a, const char* who) {
@@ -638,6 +1285,60 @@ int wfd_yee_step(const wfd_yee_plan* p, void* stream) {
       break;
   }
   WFD_LAUNCH_CHECK("yee_step");
+  return WFD_OK;
+}
+
+int wfd_yee_pass(const wfd_yee_plan* p, const wfd_yee_targets* out, void* stream) {
+  PassArgs pa;
+  if (int status = build_args(p, pa.a, "yee_pass")) return status;
+  WFD_REQUIRE(out && out->e && out->h && out->psi_e && out->psi_h && out->ticket,
+              "yee_pass: null target or ticket pointer");
+  WFD_REQUIRE(out->e != p->e && out->h != p->h && out->psi_e != p->psi_e &&
+                  out->psi_h != p->psi_h,
+              "yee_pass: targets must be buffers other than the plan's");
+  const int steps = out->steps ? out->steps : 1;
+  WFD_REQUIRE(steps == 1 || steps == 2, "yee_pass: steps must be 1 or 2");
+  // one step: 256 threads x 2 or 4 offsets; two steps: 512 threads x 1 or 2
+  const int threads = steps == 1 ? kBlock : 2 * kBlock;
+  const int per_thread = out->slots_per_thread ? out->slots_per_thread : (steps == 1 ? 4 : 2);
+  WFD_REQUIRE(steps == 1 ? (per_thread == 2 || per_thread == 4)
+                         : (per_thread == 1 || per_thread == 2),
+              "yee_pass: slots_per_thread must be 2 or 4 (one step), 1 or 2 (two steps)");
+  const int slots = per_thread * threads;
+  const int halo = steps == 1 ? p->nz : 3 * p->nz;
+  WFD_REQUIRE(halo < slots, "yee_pass: nz = %d leaves no tile in %d slots per workgroup", p->nz,
+              slots);
+  WFD_REQUIRE(out->planes >= 0 && out->planes <= 64, "yee_pass: planes must be 0..64");
+  pa.e_out = out->e;
+  pa.h_out = out->h;
+  pa.pe_out = out->psi_e;
+  pa.ph_out = out->psi_h;
+  pa.ticket = out->ticket;
+  pa.W = slots - halo;
+  pa.tiles = (pa.a.P + pa.W - 1) / pa.W;
+  int xc = out->planes;
+  if (xc == 0) {
+    // about 2048 workgroups, at least 4 planes per chunk
+    xc = (int)(((int64_t)pa.tiles * p->nx + 2047) / 2048);
+    xc = max(4, min(16, xc));
+  }
+  pa.xc = xc;
+  const int chunks = (p->nx + xc - 1) / xc;
+  pa.total = pa.tiles * chunks;
+  pa.per = (pa.total + 7) / 8;
+  hipStream_t s = as_stream(stream);
+  const dim3 grid(pa.per * 8);
+  if (steps == 2) {
+    if (per_thread == 1)
+      hipLaunchKernelGGL((k_yee_pass2<2 * kBlock, 1>), grid, dim3(threads), 0, s, pa);
+    else
+      hipLaunchKernelGGL((k_yee_pass2<2 * kBlock, 2>), grid, dim3(threads), 0, s, pa);
+  } else if (per_thread == 2) {
+    hipLaunchKernelGGL(k_yee_pass<2>, grid, dim3(kBlock), 0, s, pa);
+  } else {
+    hipLaunchKernelGGL(k_yee_pass<4>, grid, dim3(kBlock), 0, s, pa);
+  }
+  WFD_LAUNCH_CHECK("yee_pass");
   return WFD_OK;
 }
 

@@ -19,8 +19,8 @@ import numpy as np
 import torch
 
 from . import kernels as k
-from ._lib import (Handles, ReplayDesc, ResidentLayout, WFD_MAX_STAGES, YeePlan, call, ptr,
-                   release, stream)
+from ._lib import (Handles, ReplayDesc, ResidentLayout, WFD_MAX_STAGES, YeePlan, YeeTargets, call,
+                   ptr, release, stream)
 from .array import Array
 
 
@@ -149,14 +149,26 @@ class Solver:
     faster where it applies (full CPML storage, every monitor staged, no
     hprev); True requires it (ValueError where it does not apply); False
     never uses it.  Both give bitwise the same fields and detectors.
+
+    Extra keyword ``single_pass``: "auto" (default) lets ``compile`` time the
+    single-pass step (wfd_yee_pass: one launch per step, E, H and CPML psi
+    alternating between two buffer sets) against the fused pair where it
+    applies (full CPML storage, every monitor staged) and the resident path
+    is not used; True requires it; False never uses it.  Captured chunks
+    pair the passes so that the state is back in ``E``, ``H``, ``psi_e``,
+    ``psi_h`` after every chunk (an odd chunk ends with one fused step);
+    single ``launch_step`` calls use the fused pair.  Bitwise the same
+    results.
     """
 
     def __init__(self, case, device="cuda:0", graph_steps=64, cpml_storage="full",
-                 full_hprev=False, x_chunk=0, resident="auto"):
+                 full_hprev=False, x_chunk=0, resident="auto", single_pass="auto"):
         if cpml_storage not in ("full", "packed"):
             raise ValueError("cpml_storage must be full or packed")
         if resident not in ("auto", True, False):
             raise ValueError('resident must be "auto", True or False')
+        if single_pass not in ("auto", True, False):
+            raise ValueError('single_pass must be "auto", True or False')
         self.cpml_storage = cpml_storage
         self.path, self.meta, self.data = load_case(case)
         d, m = self.data, self.meta
@@ -249,6 +261,14 @@ class Solver:
         self.resident = False
         self.layout = None
         self._workspace = None
+        self.single_pass_mode = single_pass
+        self.single_pass = False
+        # (x planes per workgroup, slots per thread, steps per launch); 0 = chosen
+        self.pass_shape = (0, 0, 1)
+        self._alt = None          # the second buffer set of the single pass
+        self._ticket = None
+        self._phase = 0           # 1 while the state is in the second set (inside a capture)
+        self._schedule = []       # launch kind of each step left in the capture
         self._compiled = False
 
     # ------------------------------------------------------------ stepping
@@ -337,6 +357,9 @@ class Solver:
 
     def _step_fields(self, stage=True):
         if self.cpml_storage == "full":
+            if self._schedule:
+                self._pass_step(stage)
+                return
             k.yee_step(self._plan(stage))
             return
         common = [self.hprev, self.save_h_from, *self.shape, self.source_x, self.courant, 0]
@@ -374,6 +397,84 @@ class Solver:
         if self.cpml_storage != "full":
             k.advance(self.step)
 
+    # ------------------------------------------------------ single-pass path
+    def single_pass_applies(self):
+        """Whether wfd_yee_pass can step this solver: full CPML storage and
+        every monitor staged (a monitor recorded every step would read the
+        buffer set the step just left), nz below 1024."""
+        return (self.cpml_storage == "full" and len(self.staged()) == len(self.monitors)
+                and self.shape[2] < 1024)
+
+    def use_single_pass(self, planes=0, slots_per_thread=0, steps=1):
+        """Step captured chunks with wfd_yee_pass, ``steps`` (1 or 2) steps per
+        launch (allocates the second buffer set)."""
+        if not self.single_pass_applies():
+            raise ValueError("the single-pass step needs full CPML storage, staged monitors "
+                             "only and nz < 1024")
+        if steps not in (1, 2):
+            raise ValueError("the single pass applies 1 or 2 steps per launch")
+        if self._alt is None:
+            self._alt = {"E": Array.zeros((3, self.n), device=self.device),
+                         "H": Array.zeros((3, self.n), device=self.device),
+                         "psi_e": Array.zeros(self.psi_e.shape, device=self.device),
+                         "psi_h": Array.zeros(self.psi_h.shape, device=self.device)}
+            self._ticket = Array.zeros(1, np.int32, self.device)
+        self.pass_shape = (int(planes), int(slots_per_thread), int(steps))
+        self.single_pass = True
+
+    def _pass_schedule(self, length, two_step=True):
+        """Launch kind of each step of a captured chunk of ``length`` steps:
+        "p2" (a two-step launch) then "skip" (its second step), "p1" (one
+        step) or "fused" (wfd_yee_step in place).  Passes come in pairs, so
+        the state is back in the first set after every pair; two-step pairs
+        cover 4 steps, and chunk flushes of the detector stages (every
+        graph_steps steps) never fall inside a two-step launch."""
+        two = two_step and self.pass_shape[2] == 2 and self.graph_steps % 2 == 0
+        kinds, left = [], int(length)
+        if two:
+            while left >= 4:
+                kinds += ["p2", "skip", "p2", "skip"]
+                left -= 4
+        while left >= 2:
+            kinds += ["p1", "p1"]
+            left -= 2
+        return kinds + ["fused"] * left
+
+    def _sets(self):
+        main = {"E": self.E, "H": self.H, "psi_e": self.psi_e, "psi_h": self.psi_h}
+        return (main, self._alt) if self._phase == 0 else (self._alt, main)
+
+    def _pass_step(self, stage):
+        """The next step of the capture's schedule: a wfd_yee_pass from the
+        current set to the other (one or two steps), nothing (the second step
+        of a two-step launch) or the fused pair in place."""
+        kind = self._schedule.pop(0)
+        if kind == "skip":
+            return
+        if kind == "fused":
+            if self._phase:
+                raise RuntimeError("a fused step with the state in the second set")
+            k.yee_step(self._plan(stage))
+            return
+        src, dst = self._sets()
+        plan = self._plan(stage)
+        plan.e, plan.h = ptr(src["E"]), ptr(src["H"])
+        plan.psi_e, plan.psi_h = ptr(src["psi_e"]), ptr(src["psi_h"])
+        out = YeeTargets()
+        out.e, out.h = ptr(dst["E"]), ptr(dst["H"])
+        out.psi_e, out.psi_h = ptr(dst["psi_e"]), ptr(dst["psi_h"])
+        out.ticket = ptr(self._ticket)
+        out.steps = 2 if kind == "p2" else 1
+        if out.steps == self.pass_shape[2]:
+            out.planes, out.slots_per_thread = self.pass_shape[:2]
+        call("wfd_yee_pass", ctypes.byref(plan), ctypes.byref(out), stream())
+        self._phase ^= 1
+
+    def current_h(self):
+        """H of the state the next step starts from (the second set while a
+        captured chunk is between two passes)."""
+        return self.H if self._phase == 0 else self._alt["H"]
+
     # ------------------------------------------------- on-chip-resident path
     def resident_applies(self):
         """Whether the resident kernel can run this solver's step: full CPML
@@ -396,7 +497,9 @@ class Solver:
 
     def use_resident(self, layout=None):
         """Switch stepping to the resident kernel with the given (default:
-        chosen) layout; allocates its zeroed workspace."""
+        chosen) layout; its workspace starts zeroed (the face words carry
+        launch-epoch tags and need no clearing between launches of one
+        layout, only when the layout using them changes)."""
         if not self.resident_applies():
             raise ValueError("resident stepping needs full CPML storage, staged monitors "
                              "only and no hprev")
@@ -404,6 +507,8 @@ class Solver:
         words = int(self.layout.workspace_bytes) // 4
         if self._workspace is None or self._workspace.shape[0] != words:
             self._workspace = Array.zeros(words, np.int32, self.device)
+        else:
+            self._workspace.zero_()
         self.resident = True
 
     def _error_word(self):
@@ -449,12 +554,18 @@ class Solver:
             self._record()
         self._finish_step()
 
-    def _capture_fn(self, length, fn):
+    def _capture_fn(self, length, fn, two_step=True, stepping=True):
         """HIP graph of `length` calls of fn, captured on a library stream;
-        staged monitors replay every graph_steps steps and at the end."""
+        staged monitors replay every graph_steps steps and at the end.
+        ``stepping``: each call of fn takes one step (through _step_fields);
+        ``two_step``: the chunk may use two-step passes (not when fn needs
+        the fields between the two steps)."""
         torch.cuda.synchronize(self.device)
         # the stage rings hold graph_steps steps: longer graphs flush inside
         self._flush_every, self._pending = min(length, self.graph_steps), 0
+        self._phase = 0
+        self._schedule = (self._pass_schedule(length, two_step)
+                          if self.single_pass and stepping else [])
         s = self._handles.begin()
         ext = torch.cuda.ExternalStream(s.value, device=self.device)
         try:
@@ -466,6 +577,10 @@ class Solver:
         finally:
             graph = self._handles.end()
             self._flush_every, self._pending = 1, 0
+            ended_in, left = self._phase, len(self._schedule)
+            self._phase, self._schedule = 0, []
+        if ended_in or left:
+            raise RuntimeError("a captured chunk ended with the state in the second set")
         return graph
 
     def _capture(self, length, record):
@@ -479,6 +594,7 @@ class Solver:
         if self.cpml_storage == "full" and self.x_chunk == 0:
             self.x_chunk = tune(self)
         self.resident = False
+        self.single_pass = False
         if self.resident_mode is True:
             if not self.resident_applies():
                 raise ValueError("resident stepping needs full CPML storage, staged monitors "
@@ -488,6 +604,16 @@ class Solver:
             layout = tune_resident(self)
             if layout is not None:
                 self.use_resident(layout)
+        if self.resident:
+            return self.x_chunk
+        from .autotune import tune_pass
+
+        if self.single_pass_mode is True:
+            self.use_single_pass(*tune_pass(self, require=True))
+        elif self.single_pass_mode == "auto" and self.single_pass_applies():
+            shape = tune_pass(self)
+            if shape is not None:
+                self.use_single_pass(*shape)
         return self.x_chunk
 
     def compile(self):
@@ -496,6 +622,12 @@ class Solver:
         self.launch_step()
         torch.cuda.synchronize(self.device)
         self.tune()
+        self.capture_all()
+        return time.perf_counter() - start
+
+    def capture_all(self):
+        """(Re)capture the step graphs with the current stepping choices
+        (compile's tail: after tune, or after use_single_pass/use_resident)."""
         self.reset()
         self._destroy_graphs()
         if not self.resident:
@@ -505,7 +637,6 @@ class Solver:
                     self._quiet_graphs[length] = self._capture(length, False)
         self.reset()
         self._compiled = True
-        return time.perf_counter() - start
 
     def _windows(self):
         """Boolean per step: does any monitor record at that step?"""

@@ -1,19 +1,18 @@
 """Surrogate-assisted inverse design of the density bend (no reference counterpart).
 
-* ``surrogate_case`` writes the bend bundle the surrogate learns: modal DFT
-  planes at 8 wavelengths (1.50-1.60 um), a side x side design region.
-* ``Evaluator`` runs the solver: batched forward + adjoint labels
-  (``BatchTransmission``: T spectrum and dT/drho at the objective
-  wavelength), single forward verifications (a plain ``Solver``, resident
-  when that is faster) and single adjoint steps.
+* ``surrogate_case`` writes the bend bundle the surrogate learns: the
+  200x200x80 bend at 20 nm with a 100x100-voxel (2x2 um) design region,
+  10-cell CPML, 4000 steps, modal DFT planes at 8 wavelengths (1.50-1.60 um)
+  plus the objective's 1.55 um.
+* ``Evaluator`` runs the solver: forward + checkpointed adjoint labels (T
+  spectrum and dT/drho at 1.55 um), single forward verifications and single
+  adjoint steps.
 * ``train`` fits the FNO (fno.py) to (rho -> T spectrum, dT/drho) on the
   library's own BF16 kernels.
 * ``optimize`` runs gradient ascent on T at the objective wavelength either
   with the adjoint at every step (``surrogate=None``) or with surrogate
   proposals, each re-verified by a full FDTD forward and replaced by an
-  adjoint step when the surrogate's transmission error exceeds ``tolerance``
-  (the surrogate's T is corrected by the last verified design's: only the
-  change from it is predicted).
+  adjoint step when the surrogate's transmission error exceeds ``tolerance``.
 """
 
 import time
@@ -23,41 +22,39 @@ import torch
 
 from . import detectors
 from .cases import bend_case, design_density
-from .design import BatchTransmission, DensityMap, _design_box
+from .design import DensityMap, TransmissionProblem, _design_box
 from .solver import Solver
 
-WAVELENGTHS = np.linspace(1.50e-6, 1.60e-6, 8)
+SPECTRUM = np.linspace(1.50e-6, 1.60e-6, 8)
+OBJECTIVE = 1.55e-6
+# the DFT planes record the spectrum and then the objective's wavelength
+WAVELENGTHS = np.append(SPECTRUM, OBJECTIVE)
 
 
-def surrogate_case(path, side=100, spacing=40e-9, steps=1500, pml=8, margin=30, nz=24,
-                   pulse_duration=30e-15):
-    """Bend with a side x side voxel design region (spacing per voxel), the
-    arms and planes ``margin`` cells outside it, modal planes at 8 wavelengths."""
-    n = side + 2 * (pml + margin)
-    return bend_case(path, np.full((side, side), 0.5), shape=(n, n, nz), spacing=spacing,
+def surrogate_case(path, side=100, shape=(200, 200, 80), spacing=20e-9, steps=4000, pml=10,
+                   pulse_duration=60e-15):
+    """The 90-degree bend with a side x side voxel design region (one voxel per
+    in-plane cell) and modal DFT planes at WAVELENGTHS (cases.bend_case)."""
+    return bend_case(path, np.full((side, side), 0.5), shape=shape, spacing=spacing,
                      steps=steps, pml=pml, design=(side, side), wavelengths=list(WAVELENGTHS),
-                     wavelength=1.55e-6, pulse_duration=pulse_duration)
+                     wavelength=OBJECTIVE, pulse_duration=pulse_duration)
 
 
 def objective_index():
-    """The sampled wavelength nearest 1.55 um (the objective's)."""
-    return int(np.argmin(np.abs(WAVELENGTHS - 1.55e-6)))
+    """Index of the objective's wavelength (1.55 um) in a label's spectrum."""
+    return len(SPECTRUM)
 
 
 class Evaluator:
-    """FDTD evaluations of densities on one surrogate bundle."""
+    """FDTD evaluations of densities on one surrogate bundle: labels and
+    adjoint steps by forward + checkpointed adjoint (design.TransmissionProblem),
+    verifications by a plain forward (Solver, no checkpoints)."""
 
-    def __init__(self, case, batch=32, checkpoint_every=192, device="cuda:0", log=None):
+    def __init__(self, case, checkpoint_every=64, device="cuda:0", log=None):
         self.log = log or (lambda message: None)
         self.case = case
-        self.wavelength = float(WAVELENGTHS[objective_index()])
-        self.batch = batch
-        self.labeler = BatchTransmission([case] * batch, wavelength=self.wavelength,
-                                         checkpoint_every=checkpoint_every, device=device)
-        self.labeler.log = self.log
-        self.log(f"compiling {batch} batched adjoints")
-        self.labeler.compile()
-        self.single = self.labeler.problems[0]
+        self.problem = TransmissionProblem(case, wavelength=OBJECTIVE,
+                                           checkpoint_every=checkpoint_every, device=device)
         self.checker = Solver(case, device=device)
         detectors.attach(self.checker)
         self.checker.compile()
@@ -65,24 +62,26 @@ class Evaluator:
         self.side = self.map.plane[0]
         self.counts = {"forward": 0, "adjoint": 0, "labels": 0}
 
-    def label(self, rhos):
-        """(T spectra (n, 8), dT/drho (n, side, side)) for the densities."""
-        rhos = list(rhos)
+    def adjoint(self, rho):
+        """(T spectrum (9,), dT/drho at the objective wavelength) of one design."""
+        _, g = self.problem.value_and_grad(rho)
+        self.counts["adjoint"] += 1
+        return self.problem.adjoint.transmission_spectrum(), g.cpu().numpy()
+
+    def label(self, rhos, every=64):
+        """(T spectra (n, 9), dT/drho (n, side, side)) of the densities."""
         spectra, grads = [], []
-        for start in range(0, len(rhos), self.batch):
-            chunk = rhos[start:start + self.batch]
-            pad = chunk + [chunk[-1]] * (self.batch - len(chunk))
-            self.labeler.forward(pad)
-            g = self.labeler.backward()
-            self.log(f"labelled {min(start + self.batch, len(rhos))}/{len(rhos)} designs")
-            for k in range(len(chunk)):
-                spectra.append(self.labeler.problems[k].adjoint.transmission_spectrum())
-                grads.append(g[k].cpu().numpy())
-        self.counts["labels"] += len(rhos)
+        for k, rho in enumerate(rhos):
+            spec, g = self.adjoint(rho)
+            spectra.append(spec)
+            grads.append(g)
+            if (k + 1) % every == 0:
+                self.log(f"labelled {k + 1}/{len(rhos)} designs")
+        self.counts["labels"] += len(spectra)
         return np.array(spectra), np.array(grads)
 
     def transmission(self, rho):
-        """T spectrum by one full FDTD forward (the verification)."""
+        """T spectrum (9,) by one full FDTD forward (the verification)."""
         s = self.checker
         self.map.write(rho, s)
         s.reset()
@@ -90,17 +89,17 @@ class Evaluator:
         self.counts["forward"] += 1
         return detectors.dft_transmission(s)
 
-    def adjoint(self, rho):
-        """(T spectrum, dT/drho at the objective wavelength) by forward + adjoint."""
-        T, g = self.single.value_and_grad(rho)
-        self.counts["adjoint"] += 1
-        return self.single.adjoint.transmission_spectrum(), g.cpu().numpy()
+
+def design(seed, index, side):
+    """Blurred-random density number ``index`` of the seeded family (sigma 2
+    voxels), independent of every other index."""
+    return design_density(np.random.default_rng([int(seed), int(index)]), (side, side),
+                          blurred=True)
 
 
-def random_designs(count, side, seed=0):
-    """Seeded blurred-random densities (sigma 2 voxels)."""
-    rng = np.random.default_rng(seed)
-    return [design_density(rng, (side, side), blurred=True) for _ in range(count)]
+def random_designs(count, side, seed=0, start=0):
+    """Designs start .. start + count - 1 of the seeded family."""
+    return [design(seed, k, side) for k in range(start, start + count)]
 
 
 class Normalizer:
@@ -222,92 +221,109 @@ def _ascent(rho, g, step):
     return np.clip(rho + step * g / scale, 0.0, 1.0)
 
 
-def optimize(evaluator, rho0, steps=200, step=0.05, surrogate=None, tolerance=0.01, log=None,
-             predictor="taylor", require_improvement=True, tuner=None, anchored=True,
-             lagged=False):
-    """Gradient ascent on T at the objective wavelength.  ``surrogate`` =
-    (net, norm) proposes each step; every proposal is re-verified by a full
-    FDTD forward, and a proposal whose surrogate T is off by more than
-    ``tolerance`` (relative) is replaced by an adjoint step.
+def optimize(evaluator, rho0, steps=200, step=0.01, surrogate=None, tolerance=0.01, log=None,
+             inner=2, max_inner=16, lagged=False, target=None, tuner=None):
+    """Gradient ascent on T at the objective wavelength (largest density
+    change ``step`` per ascent step).
 
-    Gradients of a proposal step: with ``anchored`` the exact adjoint
-    gradient of the last adjoint design (the anchor) plus the network's
-    change of gradient since it, g_a + g_net(rho) - g_net(rho_a) (the first
-    step takes an adjoint step to get an anchor); else the network's g_net.
-    ``lagged`` drops the network's correction (g_a alone: the ablation
-    without a surrogate).  The surrogate's T of a proposal (``predictor``),
-    with T0 the last verified T and d = proposal - current design:
-    "taylor": T0 + <g(current) + g(proposal), d> / 2 (trapezoid rule);
-    "difference": T0 + net(proposal) - net(current) with the network's T
-    head; "absolute": net(proposal).  ``require_improvement``: a proposal
-    whose verified T is below the current one is replaced by an adjoint step
-    as well, so accepted surrogate steps never lose ground.  ``tuner`` (an
-    OnlineTuner) learns from every adjoint label the loop computes.
-    Returns a dict with the T trajectory (verified), the surrogate's relative
-    T error on every proposal, counts and wall time."""
+    ``surrogate=None``: the plain loop, an adjoint (forward + checkpointed
+    reverse) at every step, ``steps`` steps, then a forward of the final
+    design.
+
+    ``surrogate=(net, norm)``: each iteration proposes m ascent steps (m
+    starts at ``inner``) along the anchored model gradient g_a + g_net(rho) -
+    g_net(rho_a), where g_a is the adjoint gradient of the anchor design
+    rho_a; the surrogate's T of the proposal is the last verified T plus the
+    trapezoid-rule integral of that gradient along the m steps.  A full FDTD
+    forward verifies every proposal: within ``tolerance`` (relative) of the
+    prediction and above the current T, it is accepted and m doubles (up to
+    ``max_inner``); otherwise m halves and the adjoint of the current design
+    becomes the new anchor (the fallback).  A one-step proposal from a fresh
+    anchor is a plain adjoint step and is always accepted.  ``lagged`` drops
+    the network's correction (g_a alone: the method without the network).
+    ``target``: stop once the verified T reaches it.  ``tuner`` (OnlineTuner)
+    learns from every adjoint label the loop computes.
+
+    Returns a dict: verified T per iteration ("history", with the wall time of
+    each in "times"), the final design and T, the relative error of every
+    surrogate prediction, fallbacks, evaluation counts and wall time."""
     f = objective_index()
     rho = np.array(rho0, np.float64)
     start = time.perf_counter()
-    history, errors, fallbacks = [], [], 0
+    history, times, errors, fallbacks, accepted, sizes = [], [], [], 0, 0, []
+    counts0 = dict(evaluator.counts)
+
+    def now():
+        return time.perf_counter() - start
+
     if surrogate is None:
         for k in range(steps):
             spec, g = evaluator.adjoint(rho)
             history.append(float(spec[f]))
+            times.append(now())
             rho = _ascent(rho, g, step)
             if log and k % 20 == 0:
                 log(f"adjoint step {k}: T = {history[-1]:.5f}")
         final = float(evaluator.transmission(rho)[f])
+        history.append(final)
+        times.append(now())
     else:
         net, norm = surrogate
-        anchor = None  # (network gradient at the anchor, adjoint gradient there)
 
-        def gradient(pred):
-            if not anchored:
-                return pred[1]
-            if lagged:
-                return anchor[1]
-            return anchor[1] + pred[1] - anchor[0]
-
-        def adjoint_step(rho):
-            nonlocal anchor
-            spec, g = evaluator.adjoint(rho)
+        def anchor_at(design):
+            spec, g = evaluator.adjoint(design)
             if tuner is not None:
-                tuner.add(rho, spec, g)
-            anchor = (predict(net, norm, rho)[1], g)
-            return _ascent(rho, g, step)
+                tuner.add(design, spec, g)
+            g_net = None if lagged else predict(net, norm, design)[1]
+            return float(spec[f]), (np.array(design), g, g_net)
 
-        current = float(evaluator.transmission(rho)[f])
-        here = predict(net, norm, rho)
-        for _ in range(steps):
-            if anchored and anchor is None:
-                fallbacks += 1
-                proposal = adjoint_step(rho)
-                t_true = float(evaluator.transmission(proposal)[f])
-                there = predict(net, norm, proposal)
+        current, anchor = anchor_at(rho)
+        history.append(current)
+        times.append(now())
+        m = max(1, int(inner))
+        for k in range(steps):
+            if target is not None and current >= target:
+                break
+            fresh = np.array_equal(anchor[0], rho)
+
+            def model_gradient(design):
+                if lagged:
+                    return anchor[1]
+                return anchor[1] + predict(net, norm, design)[1] - anchor[2]
+
+            path, g_here, t_hat = rho, anchor[1] if fresh else model_gradient(rho), current
+            for _ in range(m):
+                nxt = _ascent(path, g_here, step)
+                g_next = model_gradient(nxt)
+                t_hat += 0.5 * float(((g_here + g_next) * (nxt - path)).sum())
+                path, g_here = nxt, g_next
+            t_true = float(evaluator.transmission(path)[f])
+            errors.append(abs(t_hat - t_true) / max(abs(t_true), 1e-12))
+            sizes.append(m)
+            if (errors[-1] <= tolerance and t_true > current) or (fresh and m == 1):
+                rho, current = path, t_true
+                accepted += 1
+                m = min(2 * m, max_inner)
             else:
-                g_here = gradient(here)
-                proposal = _ascent(rho, g_here, step)
-                there = predict(net, norm, proposal)
-                d = proposal - rho
-                if predictor == "taylor":
-                    t_hat = current + 0.5 * float(((g_here + gradient(there)) * d).sum())
-                elif predictor == "difference":
-                    t_hat = current + there[0][f] - here[0][f]
-                else:
-                    t_hat = there[0][f]
-                t_true = float(evaluator.transmission(proposal)[f])
-                errors.append(abs(t_hat - t_true) / max(abs(t_true), 1e-12))
-                if errors[-1] > tolerance or (require_improvement and t_true < current):
-                    fallbacks += 1
-                    proposal = adjoint_step(rho)
-                    t_true = float(evaluator.transmission(proposal)[f])
-                    there = predict(net, norm, proposal)
-            rho, current, here = proposal, t_true, there
+                fallbacks += 1
+                m = max(1, m // 2)
+                current, anchor = anchor_at(rho)
             history.append(current)
-            if log and len(history) % 20 == 1:
-                log(f"surrogate step {len(history) - 1}: T = {current:.5f}, fallbacks {fallbacks}")
+            times.append(now())
+            if log and k % 20 == 0:
+                log(f"surrogate iteration {k}: T = {current:.5f}, m {m}, fallbacks {fallbacks}")
         final = current
     torch.cuda.synchronize()
-    return {"final": final, "history": history, "rho": rho, "fallbacks": fallbacks,
-            "errors": errors, "seconds": time.perf_counter() - start,
-            "counts": dict(evaluator.counts)}
+    counts = {key: evaluator.counts[key] - counts0.get(key, 0) for key in evaluator.counts}
+    return {"final": final, "history": history, "times": times, "rho": rho,
+            "fallbacks": fallbacks, "accepted": accepted, "inner_sizes": sizes,
+            "errors": errors, "seconds": now(), "counts": counts}
+
+
+def time_to(result, level):
+    """Wall seconds until a loop's verified T first reached ``level`` (None if
+    it never did)."""
+    for t, value in zip(result["times"], result["history"]):
+        if value >= level:
+            return t
+    return None
