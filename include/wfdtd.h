@@ -201,6 +201,21 @@ int wfd_resident_layout_for(const wfd_yee_plan* plan, int sx, int sy, int sz,
 int wfd_resident_run(const wfd_yee_plan* plan, const wfd_resident_layout* layout,
                      void* workspace, int nsteps, int spin_limit, void* stream);
 
+/* wfd_resident_run without a workspace argument and without any allocation
+ * (no reference counterpart: for callers that cannot allocate device memory,
+ * such as code under HIP graph capture or a serving loop that forbids
+ * hipMalloc).  The face words, error word and epoch live in a 16 MiB
+ * device array the library owns, one per GPU, held until the process ends;
+ * layouts whose workspace_bytes exceed it are rejected (WFD_ERR_ARGUMENT).
+ * Each call clears the part it uses before the launch and, after it,
+ * writes the error word to *status (a device int of the caller: 0, or
+ * nonzero when a bounded poll gave up and the fields of that run are
+ * invalid); with nsteps == 0 nothing is launched and *status is left as
+ * is.  Every call on one GPU shares that array: calls that overlap on one
+ * GPU (concurrent streams, other threads) race. */
+int wfd_resident_run_noalloc(const wfd_yee_plan* plan, const wfd_resident_layout* layout,
+                             int nsteps, int spin_limit, int* status, void* stream);
+
 /* ------------------------------------------------------------- detectors
  * warp_fdtd/detectors.py:65-106 record(): one monitor, one launch over its
  * plane.  weights (steps, 2) float32 complex; phasor (6, count, 2) float32;

@@ -139,6 +139,8 @@ SIGNATURES = {
     "wfd_batch_step": [ctypes.POINTER(YeePlan), I, P, P],
     "wfd_resident_layout_for": [ctypes.POINTER(YeePlan), I, I, I, ctypes.POINTER(ResidentLayout)],
     "wfd_resident_run": [ctypes.POINTER(YeePlan), ctypes.POINTER(ResidentLayout), P, I, I, P],
+    "wfd_resident_run_noalloc": [ctypes.POINTER(YeePlan), ctypes.POINTER(ResidentLayout), I, I, P,
+                                 P],
     "wfd_record": [P, P, P, P, P, P, P, P, P] + [I] * 13 + [P],
     "wfd_record_monitors": [P, P, P, P, ctypes.POINTER(MonitorDesc), I, I, I, I, P],
     "wfd_replay_monitors": [ctypes.POINTER(ReplayDesc), I, I, P, I, I, I, I, P],

@@ -35,6 +35,8 @@
 // times out raises the error word, which every poller checks, and its
 // workgroup leaves; the launch is cooperative, which guarantees co-residency
 // of the whole grid or fails.
+#include <mutex>
+
 #include "common.hpp"
 
 namespace wfd {
@@ -636,6 +638,36 @@ __global__ void k_fill_zero(unsigned* w, size_t count) {
     w[i] = 0u;
 }
 
+// Scratch of wfd_resident_run_noalloc: face words, error word and epoch of
+// one launch, in device memory that the library owns (one copy per GPU,
+// held until the process ends).  Host code never passes `scratch` itself to
+// a kernel (in host code the name is a stand-in, not a device address): it
+// uses the address hipGetSymbolAddress returns on each device.
+constexpr size_t kScratchBytes = size_t(16) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up on the first call
+// made with that device current.
+int scratch_address(void** out) {
+  int device = 0;
+  WFD_HIP(hipGetDevice(&device));
+  WFD_REQUIRE(device >= 0 && device < kMaxDevices, "resident_run_noalloc: device %d beyond %d",
+              device, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    WFD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    WFD_REQUIRE(p != nullptr, "resident_run_noalloc: no device address for the scratch array");
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return WFD_OK;
+}
+
 struct Layout {
   int sx, sy, sz, gx, gy, gz, blocks;
 };
@@ -736,8 +768,9 @@ namespace wfd {
 namespace {
 
 // Validate and launch one resident chunk with its face words, error word and
-// epoch at `workspace` (`clear`: zeroed first, for a workspace in an unknown
-// state).  `status` (optional) gets the error word after the launch.
+// epoch at `workspace` (wfd_resident_run: the caller's; the no-allocation
+// variant: the library's scratch, zeroed first).  `status` (optional) gets
+// the error word after the launch.
 int resident_launch(const wfd_yee_plan* p, const wfd_resident_layout* L, void* workspace,
                     bool clear, int nsteps, int spin_limit, int* status, void* stream) {
   WFD_REQUIRE(p && L && workspace, "resident_run: null argument");
@@ -834,6 +867,19 @@ extern "C" {
 int wfd_resident_run(const wfd_yee_plan* p, const wfd_resident_layout* L, void* workspace,
                      int nsteps, int spin_limit, void* stream) {
   return resident_launch(p, L, workspace, false, nsteps, spin_limit, nullptr, stream);
+}
+
+int wfd_resident_run_noalloc(const wfd_yee_plan* p, const wfd_resident_layout* L, int nsteps,
+                             int spin_limit, int* status, void* stream) {
+  WFD_REQUIRE(p && L && status, "resident_run_noalloc: null argument");
+  WFD_REQUIRE(L->workspace_bytes > 0 && (size_t)L->workspace_bytes <= kScratchBytes,
+              "resident_run_noalloc: the layout needs %lld bytes of face words, more than the "
+              "%zu-byte scratch (use wfd_resident_run with a caller workspace)",
+              (long long)L->workspace_bytes, kScratchBytes);
+  void* ws = nullptr;
+  const int rc = scratch_address(&ws);
+  if (rc != WFD_OK) return rc;
+  return resident_launch(p, L, ws, true, nsteps, spin_limit, status, stream);
 }
 
 }  // extern "C"
