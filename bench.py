@@ -401,13 +401,14 @@ def bench_surrogate_label(args):
           check=check)
 
 
-def load_labels(directory):
-    """Label files written by --workload surrogate-label in a directory:
-    (designs, spectra (n, 9), dT/drho (n, 100, 100)), ordered by first index."""
+def load_labels(directories):
+    """Label files written by --workload surrogate-label in one or more
+    directories (comma-separated): (designs, spectra (n, 9), dT/drho (n, 100,
+    100)), ordered by seed and first index."""
     import __graft_entry__
 
     sg = __graft_entry__.package().surrogate
-    files = sorted(Path(directory).glob("*.npz"))
+    files = sorted(f for d in directories.split(",") if d for f in Path(d).glob("*.npz"))
     parts = []
     for f in files:
         with np.load(f) as d:
@@ -453,6 +454,7 @@ def bench_surrogate(args):
                           log=log)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
+    net = sg.inference_copy(net)
     f = sg.objective_index()
     pred = np.array([sg.predict(net, norm, r)[0] for r in held_rho])
     t_err = np.abs(pred[:, f] - T_held[:, f])
@@ -544,7 +546,8 @@ def main():
                    help="waveguide: the headline Yee-step metric; the others measure the "
                         "adjoint, batched-design and surrogate workloads")
     p.add_argument("--fd", action="store_true", help="bend-adjoint: central-difference check")
-    p.add_argument("--labels", default="", help="surrogate: directory of label files")
+    p.add_argument("--labels", default="",
+                   help="surrogate: directories of label files (comma-separated)")
     p.add_argument("--held-out", type=int, default=64,
                    help="surrogate: last labels kept out of training")
     p.add_argument("--epochs", type=int, default=40, help="surrogate: training epochs")

@@ -147,11 +147,16 @@ typedef struct wfd_yee_targets {
   float* psi_h;
   int* ticket;
   int planes;           /* x planes per workgroup (0: chosen) */
-  int slots_per_thread; /* one step: 2 or 4 (0: 4), a 256-thread workgroup
-                           covering 256 x this - nz (y, z) cells; two steps:
-                           1 or 2 (0: 2), 512 threads covering 512 x this -
-                           3 nz cells */
+  int slots_per_thread; /* one step: 1 (512 threads) or 2 or 4 (256 threads;
+                           0: 4), a workgroup covering threads x this - nz
+                           (y, z) cells; two steps: 1 or 2 (0: 2), 512
+                           threads covering 512 x this - 3 nz cells */
   int steps;            /* steps per launch: 1 or 2 (0: 1) */
+  float* hist;          /* two steps only, optional: the adjoint curl history
+                           of both steps, written as wfd_record_curl would
+                           (slot step % hist_steps, box below) */
+  int hist_steps;
+  int hist_box[6];      /* x0, y0, z0, sx, sy, sz */
 } wfd_yee_targets;
 
 int wfd_yee_pass(const wfd_yee_plan* plan, const wfd_yee_targets* out, void* stream);

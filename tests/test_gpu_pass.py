@@ -73,8 +73,8 @@ def test_forward50_run_bitwise(pkg, gpu, steps):
     assert int(b.step.numpy()[0]) == b.steps
 
 
-@pytest.mark.parametrize("shape", [(4, 2, 1), (16, 4, 1), (5, 4, 1), (8, 2, 2), (4, 1, 2),
-                                   (13, 2, 2)])
+@pytest.mark.parametrize("shape", [(4, 2, 1), (16, 4, 1), (5, 4, 1), (2, 1, 1), (1, 1, 1),
+                                   (8, 2, 2), (4, 1, 2), (13, 2, 2), (1, 2, 2)])
 def test_launch_shapes_and_odd_chunks_bitwise(pkg, gpu, bend, shape):
     """Planes per workgroup x slots per thread x steps per launch, each over
     ragged run_steps (lengths of 4k + 1..3 end with one-step passes and one
@@ -111,17 +111,18 @@ def test_bend_against_the_oracle(pkg, gpu, bend):
     s.close()
 
 
-@pytest.mark.parametrize("steps", [1, 2])
-def test_checkpointed_adjoint_on_the_single_pass_bitwise(pkg, gpu, bend, steps):
-    """The adjoint's forward (two steps per launch where no curl is
-    recorded), checkpoints and recompute (one step per launch: record_curl
-    reads H from whichever set holds the step's start): T and dT/drho
-    bitwise those of the fused pair."""
+@pytest.mark.parametrize("steps,checkpoint", [(1, 64), (2, 64), (2, None)])
+def test_checkpointed_adjoint_on_the_single_pass_bitwise(pkg, gpu, bend, steps, checkpoint):
+    """The adjoint's forward, checkpoints and recompute on one- or two-step
+    passes (one-step chunks: record_curl reads H from whichever set holds
+    the step's start; two-step passes write the curl history of both their
+    steps themselves), with checkpoints every 64 steps or the full history:
+    T and dT/drho bitwise those of the fused pair."""
     rho = pkg.surrogate.random_designs(1, 100, seed=5)[0]
     out = []
     for single in (False, True):
         p = pkg.design.TransmissionProblem(bend, wavelength=pkg.surrogate.OBJECTIVE,
-                                           checkpoint_every=64, single_pass=single)
+                                           checkpoint_every=checkpoint, single_pass=single)
         p._compile()
         assert p.adjoint.single_pass == single
         if single:

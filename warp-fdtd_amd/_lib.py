@@ -56,7 +56,8 @@ class YeeTargets(ctypes.Structure):
     """wfd_yee_targets (include/wfdtd.h)."""
 
     _fields_ = [("e", P), ("h", P), ("psi_e", P), ("psi_h", P), ("ticket", P), ("planes", I),
-                ("slots_per_thread", I), ("steps", I)]
+                ("slots_per_thread", I), ("steps", I), ("hist", P), ("hist_steps", I),
+                ("hist_box", I * 6)]
 
 
 class InjectSource(ctypes.Structure):

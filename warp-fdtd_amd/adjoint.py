@@ -169,13 +169,17 @@ class Adjoint(Solver):
     def launch_step(self, record=True, curl=True):
         """adjoint.py:271-327: record_curl reads H before this step's H
         half-step, i.e. before the fused E+H step as well."""
-        if curl:
+        if curl and self._schedule and self._schedule[0] in ("p2", "skip"):
+            # a two-step pass records the curl of both its steps itself
+            self._curl = (self.history, self.history.shape[0], self.box)
+        elif curl:
             x0, y0, z0, sx, sy, sz = self.box
             call("wfd_record_curl", ptr(self.current_h()), ptr(self.history),
                  int(self.history.shape[0]),
                  ptr(self.step), *self.shape, x0, y0, z0, sx, sy, sz, float(self.courant),
                  stream())
         self._step_fields(stage=record)
+        self._curl = None
         if record:
             self._record()
         self._finish_step()
@@ -200,17 +204,16 @@ class Adjoint(Solver):
             lengths |= {K, self.steps % K} - {0}
         for length in lengths:
             # forward with monitors; with checkpoints the curl history is only
-            # recorded during recomputation (the curl history needs H between
-            # every two steps: one-step passes there)
+            # recorded during recomputation (two-step passes record it themselves)
             self.graphs[length] = self._capture_fn(
-                length, lambda: self.launch_step(record=True, curl=full), two_step=not full)
+                length, lambda: self.launch_step(record=True, curl=full))
             self._quiet_graphs[length] = self._capture_fn(
-                length, lambda: self.launch_step(record=False, curl=full), two_step=not full)
+                length, lambda: self.launch_step(record=False, curl=full))
         self._recompute_graphs = {}
         if not full:
             for length in {self.checkpoint_every, self.steps % self.checkpoint_every} - {0}:
                 self._recompute_graphs[length] = self._capture_fn(
-                    length, lambda: self.launch_step(record=False, curl=True), two_step=False)
+                    length, lambda: self.launch_step(record=False, curl=True))
         self.reset()
         self._compiled = True
 

@@ -184,12 +184,13 @@ def tune_resident(solver, use_cache=True, require=False):
     return best if (use or require) else None
 
 
-PASS_PLANES = (1, 2, 4, 8)
-PASS_SHAPES = {1: (2, 4), 2: (1, 2)}  # steps per launch -> slots per thread
+# steps per launch -> (x planes per workgroup, slots per thread) candidates
+PASS_PLANES = {1: (1, 2, 4, 8), 2: (2, 4, 8, 16, 32)}
+PASS_SHAPES = {1: (1, 2, 4), 2: (1, 2)}
 
 
 def _pass_fits(nz, steps, slots):
-    threads = 256 if steps == 1 else 512
+    threads = 256 if (steps == 1 and slots > 1) else 512
     return (nz if steps == 1 else 3 * nz) < slots * threads
 
 
@@ -223,7 +224,7 @@ def tune_pass(solver, use_cache=True, require=False):
     """(planes, slots per thread, steps per launch) of the fastest single-pass
     shape if it beats the tuned fused step on this grid (or in any case with
     ``require``), else None."""
-    key = _key(solver) + "|single-pass-3"
+    key = _key(solver) + "|single-pass-4"
     if use_cache:
         hit = _memory.get(key)
         if hit is None:
@@ -233,7 +234,7 @@ def tune_pass(solver, use_cache=True, require=False):
             return tuple(hit["shape"]) if (hit["single_pass"] or require) else None
     nz = solver.shape[2]
     shapes = [(p, j, k) for k, slots in PASS_SHAPES.items() for j in slots
-              if _pass_fits(nz, k, j) for p in PASS_PLANES]
+              if _pass_fits(nz, k, j) for p in PASS_PLANES[k]]
     if not shapes:
         if require:
             raise ValueError(f"no single-pass shape fits nz = {nz}")

@@ -501,7 +501,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_h(const YeeArgs* __restrict__ 
 // is recomputed the same way.  Each cell's E, H and 1/eps are read once and
 // its E and H written once (52 B per cell-update plus CPML psi); the halo
 // rows and planes are re-read by the neighbouring workgroups through L2.
-// Thread t of the block handles the offsets A + t + kBlock * j, j < J.
+// Thread t of a T-thread block handles the offsets A + t + T * j, j < J.
 struct PassArgs {
   YeeArgs a;     // the step's start (e, h, psi_e, psi_h) and every table
   float* e_out;  // the step's result
@@ -509,6 +509,9 @@ struct PassArgs {
   float* pe_out;
   float* ph_out;
   int* ticket;   // workgroups finished (the last one advances the counters)
+  float* hist;   // adjoint curl history (null: none), as wfd_record_curl writes it
+  int hist_steps;
+  int hbox[6];   // its box x0, y0, z0, sx, sy, sz
   int W;         // tile cells (W + nz <= J * kBlock)
   int tiles;     // tiles per plane
   int xc;        // x planes per chunk
@@ -516,9 +519,9 @@ struct PassArgs {
   int per;       // logical workgroups per XCD stripe
 };
 
-template <int J>
-__global__ __launch_bounds__(kBlock) void k_yee_pass(PassArgs pa) {
-  constexpr int SLOTS = J * kBlock;
+template <int T, int J>
+__global__ __launch_bounds__(T) void k_yee_pass(PassArgs pa) {
+  constexpr int SLOTS = J * T;
   __shared__ float lds[2][3][SLOTS];  // the new E of planes x - 1 and x
   const YeeArgs& a = pa.a;
   const int logical = (blockIdx.x & 7) * pa.per + (blockIdx.x >> 3);
@@ -552,7 +555,7 @@ __global__ __launch_bounds__(kBlock) void k_yee_pass(PassArgs pa) {
     float ep[J][3];  // new E at the plane before
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-      o[j] = A + threadIdx.x + kBlock * j;
+      o[j] = A + threadIdx.x + T * j;
       const int oc = o[j] < Bx ? o[j] : A;
       y[j] = div_small(oc, nz, a.inv_nz);
       z[j] = oc - y[j] * nz;
@@ -669,7 +672,7 @@ __global__ __launch_bounds__(kBlock) void k_yee_pass(PassArgs pa) {
               E1[2 * n + i] = vz;
             }
           }
-          const int s = threadIdx.x + kBlock * j;
+          const int s = threadIdx.x + T * j;
           cur[s] = en[j][0];
           cur[SLOTS + s] = en[j][1];
           cur[2 * SLOTS + s] = en[j][2];
@@ -701,7 +704,7 @@ __global__ __launch_bounds__(kBlock) void k_yee_pass(PassArgs pa) {
         for (int j = 0; j < J; ++j) {
           if (o[j] < B) {
             const int i = xh * P + o[j];
-            const int s = threadIdx.x + kBlock * j;
+            const int s = threadIdx.x + T * j;
             const bool up_y = y[j] + 1 < ny, up_z = z[j] + 1 < nz;
             const float qx = up_y ? prev[s + nz] : 0.f;
             const float qz = up_y ? prev[2 * SLOTS + s + nz] : 0.f;
@@ -892,6 +895,59 @@ __device__ __forceinline__ void store_psi6(float* __restrict__ psi, int n, int i
   }
 }
 
+// The step's-start values one new-E computation of the two-step pass reads
+// at a cell: H and its y - 1 / z - 1 neighbours, E, 1/eps and the E psi.
+struct PlaneIn {
+  float h[3], yx, yz, zx, zy, e[3], ie, ps[6];
+};
+
+__device__ __forceinline__ void load_plane(const YeeArgs& a, int x, int o, int y, int z,
+                                           bool on, PlaneIn& q) {
+  const float* __restrict__ H0 = a.h;
+  const float* __restrict__ E0 = a.e;
+  const int n = a.n, nz = a.nz;
+  q.h[0] = q.h[1] = q.h[2] = q.yx = q.yz = q.zx = q.zy = 0.f;
+  q.e[0] = q.e[1] = q.e[2] = q.ie = 0.f;
+  for (int c = 0; c < 6; ++c) q.ps[c] = 0.f;
+  if (!on) return;
+  const int i = x * a.P + o;
+  q.h[0] = H0[i];
+  q.h[1] = H0[n + i];
+  q.h[2] = H0[2 * n + i];
+  if (y > 0) {
+    q.yx = H0[i - nz];
+    q.yz = H0[2 * n + i - nz];
+  }
+  if (z > 0) {
+    q.zx = H0[i - 1];
+    q.zy = H0[n + i - 1];
+  }
+  q.e[0] = E0[i];
+  q.e[1] = E0[n + i];
+  q.e[2] = E0[2 * n + i];
+  q.ie = a.inv_eps[i];
+  load_psi6(a.psi_e, n, i, rows_at(a.coeff_e, a.rows, x, y, z), q.ps);
+}
+
+// wfd_record_curl's value (adjoint.py:17-43) of one cell at the start of step
+// t from the differences of H the new-E update uses, when the cell is in the
+// history box.
+__device__ __forceinline__ void record_curl_at(const PassArgs& pa, int t, int x, int y, int z,
+                                               float dx1, float dx2, float dy0, float dy2,
+                                               float dz0, float dz1) {
+  const int* b = pa.hbox;
+  if (x < b[0] || y < b[1] || z < b[2] || x >= b[0] + b[3] || y >= b[1] + b[4] ||
+      z >= b[2] + b[5])
+    return;
+  const float S = pa.a.courant;
+  const int dn = b[3] * b[4] * b[5];
+  const int j = ((x - b[0]) * b[4] + (y - b[1])) * b[5] + (z - b[2]);
+  float* h = pa.hist + (int64_t)(t % pa.hist_steps) * 3 * dn + j;
+  h[0] = S * (dy2 - dz1);
+  h[dn] = S * (dz0 - dx2);
+  h[2 * dn] = S * (dx1 - dy0);
+}
+
 template <int T, int J>
 __global__ __launch_bounds__(T) void k_yee_pass2(PassArgs pa) {
   constexpr int SLOTS = J * T;
@@ -947,6 +1003,12 @@ __global__ __launch_bounds__(T) void k_yee_pass2(PassArgs pa) {
 #pragma unroll 1
     for (int x = xs; x <= x1 + 1; ++x) {
       float en[J][3], hn[J][3], pse[J][6], iex[J], h1n[J][3], psh[J][6], e2n[J][3];
+      // every load of the plane first (loading the next plane one iteration
+      // ahead measured slower: more registers, fewer waves)
+      PlaneIn in[J];
+#pragma unroll
+      for (int j = 0; j < J; ++j)
+        load_plane(a, x, o[j], y[j], z[j], x < nx && o[j] >= 0 && o[j] < Be, in[j]);
       // (a) E1 of plane x over [A - nz, B + 2 nz)
 #pragma unroll
       for (int j = 0; j < J; ++j) {
@@ -954,24 +1016,20 @@ __global__ __launch_bounds__(T) void k_yee_pass2(PassArgs pa) {
         for (int c = 0; c < 6; ++c) pse[j][c] = 0.f;
         iex[j] = 0.f;
         if (x < nx && o[j] >= 0 && o[j] < Be) {
-          const int i = x * P + o[j];
           const Rows r = rows_at(ce, rows, x, y[j], z[j]);
-          hn[j][0] = H0[i];
-          hn[j][1] = H0[n + i];
-          hn[j][2] = H0[2 * n + i];
-          const float yx = y[j] > 0 ? H0[i - nz] : 0.f;
-          const float yz = y[j] > 0 ? H0[2 * n + i - nz] : 0.f;
-          const float zx = z[j] > 0 ? H0[i - 1] : 0.f;
-          const float zy = z[j] > 0 ? H0[n + i - 1] : 0.f;
-          const float ex = E0[i], ey = E0[n + i], ez = E0[2 * n + i];
-          iex[j] = a.inv_eps[i];
-          load_psi6(a.psi_e, n, i, r, pse[j]);
+          for (int c = 0; c < 3; ++c) hn[j][c] = in[j].h[c];
+          for (int c = 0; c < 6; ++c) pse[j][c] = in[j].ps[c];
+          iex[j] = in[j].ie;
+          const float dx1 = hn[j][1] - hp0[j][1], dx2 = hn[j][2] - hp0[j][2];
+          const float dy0 = hn[j][0] - in[j].yx, dy2 = hn[j][2] - in[j].yz;
+          const float dz0 = hn[j][0] - in[j].zx, dz1 = hn[j][1] - in[j].zy;
+          if (pa.hist && x >= x0 && x < x1 && o[j] >= A && o[j] < B)
+            record_curl_at(pa, t, x, y[j], z[j], dx1, dx2, dy0, dy2, dz0, dz1);
           float dxz, dxy, dyx, dyz, dzy, dzx;
-          partials(r, hn[j][1] - hp0[j][1], hn[j][2] - hp0[j][2], hn[j][0] - yx, hn[j][2] - yz,
-                   hn[j][0] - zx, hn[j][1] - zy, pse[j], dxz, dxy, dyx, dyz, dzy, dzx);
-          float vx = ex + (S * (dyz - dzy)) * iex[j];
-          float vy = ey + (S * (dzx - dxz)) * iex[j];
-          float vz = ez + (S * (dxy - dyx)) * iex[j];
+          partials(r, dx1, dx2, dy0, dy2, dz0, dz1, pse[j], dxz, dxy, dyx, dyz, dzy, dzx);
+          float vx = in[j].e[0] + (S * (dyz - dzy)) * iex[j];
+          float vy = in[j].e[1] + (S * (dzx - dxz)) * iex[j];
+          float vz = in[j].e[2] + (S * (dxy - dyx)) * iex[j];
           if (src1 && x == a.source_x) {
             vy += ((a.src_h[2 * P + o[j]] * wh1y) * S) * iex[j];
             vz -= ((a.src_h[P + o[j]] * wh1z) * S) * iex[j];
@@ -1038,10 +1096,13 @@ __global__ __launch_bounds__(T) void k_yee_pass2(PassArgs pa) {
           const float yz = y[j] > 0 ? lh1[2][s - nz] : 0.f;
           const float zx = z[j] > 0 ? lh1[0][s - 1] : 0.f;
           const float zy = z[j] > 0 ? lh1[1][s - 1] : 0.f;
+          const float dx1 = h1n[j][1] - h1p[j][1], dx2 = h1n[j][2] - h1p[j][2];
+          const float dy0 = h1n[j][0] - yx, dy2 = h1n[j][2] - yz;
+          const float dz0 = h1n[j][0] - zx, dz1 = h1n[j][1] - zy;
+          if (pa.hist && xh < x1 && o[j] < B)
+            record_curl_at(pa, t + 1, xh, y[j], z[j], dx1, dx2, dy0, dy2, dz0, dz1);
           float dxz, dxy, dyx, dyz, dzy, dzx;
-          partials(r, h1n[j][1] - h1p[j][1], h1n[j][2] - h1p[j][2], h1n[j][0] - yx,
-                   h1n[j][2] - yz, h1n[j][0] - zx, h1n[j][1] - zy, pse1[j], dxz, dxy, dyx, dyz,
-                   dzy, dzx);
+          partials(r, dx1, dx2, dy0, dy2, dz0, dz1, pse1[j], dxz, dxy, dyx, dyz, dzy, dzx);
           float vx = e1p[j][0] + (S * (dyz - dzy)) * ie1[j];
           float vy = e1p[j][1] + (S * (dzx - dxz)) * ie1[j];
           float vz = e1p[j][2] + (S * (dxy - dyx)) * ie1[j];
@@ -1298,12 +1359,13 @@ int wfd_yee_pass(const wfd_yee_plan* p, const wfd_yee_targets* out, void* stream
               "yee_pass: targets must be buffers other than the plan's");
   const int steps = out->steps ? out->steps : 1;
   WFD_REQUIRE(steps == 1 || steps == 2, "yee_pass: steps must be 1 or 2");
-  // one step: 256 threads x 2 or 4 offsets; two steps: 512 threads x 1 or 2
-  const int threads = steps == 1 ? kBlock : 2 * kBlock;
+  // one step: 512 threads x 1 offset or 256 threads x 2 or 4; two steps:
+  // 512 threads x 1 or 2
   const int per_thread = out->slots_per_thread ? out->slots_per_thread : (steps == 1 ? 4 : 2);
-  WFD_REQUIRE(steps == 1 ? (per_thread == 2 || per_thread == 4)
+  WFD_REQUIRE(steps == 1 ? (per_thread == 1 || per_thread == 2 || per_thread == 4)
                          : (per_thread == 1 || per_thread == 2),
-              "yee_pass: slots_per_thread must be 2 or 4 (one step), 1 or 2 (two steps)");
+              "yee_pass: slots_per_thread must be 1, 2 or 4 (one step), 1 or 2 (two steps)");
+  const int threads = (steps == 2 || per_thread == 1) ? 2 * kBlock : kBlock;
   const int slots = per_thread * threads;
   const int halo = steps == 1 ? p->nz : 3 * p->nz;
   WFD_REQUIRE(halo < slots, "yee_pass: nz = %d leaves no tile in %d slots per workgroup", p->nz,
@@ -1314,6 +1376,17 @@ int wfd_yee_pass(const wfd_yee_plan* p, const wfd_yee_targets* out, void* stream
   pa.pe_out = out->psi_e;
   pa.ph_out = out->psi_h;
   pa.ticket = out->ticket;
+  pa.hist = out->hist;
+  pa.hist_steps = out->hist_steps > 0 ? out->hist_steps : 1;
+  for (int k = 0; k < 6; ++k) pa.hbox[k] = out->hist_box[k];
+  if (out->hist) {
+    WFD_REQUIRE(steps == 2, "yee_pass: the curl history is recorded by two-step passes only");
+    WFD_REQUIRE(out->hist_steps > 0, "yee_pass: hist_steps must be positive");
+    const int* b = out->hist_box;
+    WFD_REQUIRE(b[0] >= 0 && b[1] >= 0 && b[2] >= 0 && b[3] >= 0 && b[4] >= 0 && b[5] >= 0 &&
+                    b[0] + b[3] <= p->nx && b[1] + b[4] <= p->ny && b[2] + b[5] <= p->nz,
+                "yee_pass: history box outside the grid");
+  }
   pa.W = slots - halo;
   pa.tiles = (pa.a.P + pa.W - 1) / pa.W;
   int xc = out->planes;
@@ -1333,10 +1406,12 @@ int wfd_yee_pass(const wfd_yee_plan* p, const wfd_yee_targets* out, void* stream
       hipLaunchKernelGGL((k_yee_pass2<2 * kBlock, 1>), grid, dim3(threads), 0, s, pa);
     else
       hipLaunchKernelGGL((k_yee_pass2<2 * kBlock, 2>), grid, dim3(threads), 0, s, pa);
+  } else if (per_thread == 1) {
+    hipLaunchKernelGGL((k_yee_pass<2 * kBlock, 1>), grid, dim3(threads), 0, s, pa);
   } else if (per_thread == 2) {
-    hipLaunchKernelGGL(k_yee_pass<2>, grid, dim3(kBlock), 0, s, pa);
+    hipLaunchKernelGGL((k_yee_pass<kBlock, 2>), grid, dim3(threads), 0, s, pa);
   } else {
-    hipLaunchKernelGGL(k_yee_pass<4>, grid, dim3(kBlock), 0, s, pa);
+    hipLaunchKernelGGL((k_yee_pass<kBlock, 4>), grid, dim3(threads), 0, s, pa);
   }
   WFD_LAUNCH_CHECK("yee_pass");
   return WFD_OK;

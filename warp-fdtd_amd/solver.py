@@ -269,6 +269,7 @@ class Solver:
         self._ticket = None
         self._phase = 0           # 1 while the state is in the second set (inside a capture)
         self._schedule = []       # launch kind of each step left in the capture
+        self._curl = None         # (history, slots, box) a two-step pass records
         self._compiled = False
 
     # ------------------------------------------------------------ stepping
@@ -467,6 +468,11 @@ class Solver:
         out.steps = 2 if kind == "p2" else 1
         if out.steps == self.pass_shape[2]:
             out.planes, out.slots_per_thread = self.pass_shape[:2]
+        if kind == "p2" and self._curl is not None:
+            hist, slots, box = self._curl
+            out.hist, out.hist_steps = ptr(hist), int(slots)
+            for q, v in enumerate(box):
+                out.hist_box[q] = int(v)
         call("wfd_yee_pass", ctypes.byref(plan), ctypes.byref(out), stream())
         self._phase ^= 1
 

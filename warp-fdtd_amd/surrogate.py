@@ -205,6 +205,18 @@ class OnlineTuner:
             self.fit.step(x, tn, gn, idx, self.lr)
 
 
+def inference_copy(net):
+    """A batch-1 FNO holding the trained net's parameters, for the loop's
+    one-design predictions (the training net evaluates net.B designs)."""
+    from .fno import FNO
+
+    one = FNO(net.S, width=net.W, modes=net.m, hidden=net.H, outputs=net.F, layers=net.L,
+              batch=1, device=net.device)
+    one.params.copy_(net.params)
+    one.sync_bf16()
+    return one
+
+
 def predict(net, norm, rho):
     """(T spectrum (8,), dT/drho (side, side)) of one density."""
     x = torch.as_tensor(np.asarray(rho, np.float32), device=net.device)
@@ -237,9 +249,10 @@ def optimize(evaluator, rho0, steps=200, step=0.01, surrogate=None, tolerance=0.
     trapezoid-rule integral of that gradient along the m steps.  A full FDTD
     forward verifies every proposal: within ``tolerance`` (relative) of the
     prediction and above the current T, it is accepted and m doubles (up to
-    ``max_inner``); otherwise m halves and the adjoint of the current design
-    becomes the new anchor (the fallback).  A one-step proposal from a fresh
-    anchor is a plain adjoint step and is always accepted.  ``lagged`` drops
+    ``max_inner``); otherwise m halves and, unless the anchor already is the
+    current design, the adjoint of the current design becomes the new anchor
+    (the fallback).  A one-step proposal from a fresh anchor is a plain
+    adjoint step and is always accepted.  ``lagged`` drops
     the network's correction (g_a alone: the method without the network).
     ``target``: stop once the verified T reaches it.  ``tuner`` (OnlineTuner)
     learns from every adjoint label the loop computes.
@@ -305,9 +318,10 @@ def optimize(evaluator, rho0, steps=200, step=0.01, surrogate=None, tolerance=0.
                 accepted += 1
                 m = min(2 * m, max_inner)
             else:
-                fallbacks += 1
                 m = max(1, m // 2)
-                current, anchor = anchor_at(rho)
+                if not fresh:  # (an anchor at this design already holds its gradient)
+                    fallbacks += 1
+                    current, anchor = anchor_at(rho)
             history.append(current)
             times.append(now())
             if log and k % 20 == 0:
