@@ -550,7 +550,7 @@ def main():
                    help="surrogate: directories of label files (comma-separated)")
     p.add_argument("--held-out", type=int, default=64,
                    help="surrogate: last labels kept out of training")
-    p.add_argument("--epochs", type=int, default=40, help="surrogate: training epochs")
+    p.add_argument("--epochs", type=int, default=80, help="surrogate: training epochs")
     p.add_argument("--opt-steps", type=int, default=200, help="surrogate: optimisation steps")
     p.add_argument("--opt-step", type=float, default=0.01,
                    help="surrogate: largest density change of a step")
