@@ -125,13 +125,15 @@ def test_elementwise_and_reductions(pkg, gpu):
     np.testing.assert_array_equal(w16.cpu().float().numpy(), wd.cpu().to(torch.bfloat16).float().numpy())
 
 
-def test_fno_forward_and_backward_match_float64_oracle(pkg, gpu):
+@pytest.mark.parametrize("S,B,outputs", [(32, 3, 8), (100, 2, 9)])
+def test_fno_forward_and_backward_match_float64_oracle(pkg, gpu, S, B, outputs):
     """The GPU network (BF16 products, FP32 accumulation, hand-written
     backward) against the float64 CPU oracle (torch FFTs + autograd) with the
-    same BF16-rounded weights: outputs and every parameter gradient."""
+    same BF16-rounded weights: outputs and every parameter gradient, on a
+    small grid and on the surrogate's 100x100 design grid with its 9
+    transmission outputs."""
     from oracle import fno as OF
-    S, B = 32, 3
-    net = pkg.fno.FNO(S, batch=B, seed=3)
+    net = pkg.fno.FNO(S, batch=B, outputs=outputs, seed=3)
     g = torch.Generator().manual_seed(4)
     rho = torch.rand(B, S, S, generator=g)
     d_t = torch.randn(B, net.F, generator=g)

@@ -96,3 +96,38 @@ def test_surrogate_loop_is_verified_and_improves(pkg, gpu, setup):
     # sums replayed in other chunks)
     assert abs(final - out["final"]) <= 1e-12 * abs(final)
     assert final > start
+
+
+@pytest.fixture(scope="module")
+def setup100(tmp_path_factory, pkg):
+    """The surrogate's own geometry (200x200x80 bend at 20 nm, 100x100-voxel
+    design, 9-wavelength modal planes) on a shortened 1000-step run: 48
+    labels and a briefly trained 100x100 FNO with the workload's 9 outputs."""
+    sg = pkg.surrogate
+    case = sg.surrogate_case(tmp_path_factory.mktemp("surrogate100") / "bend", steps=1000)
+    ev = sg.Evaluator(case, checkpoint_every=64)
+    rhos = sg.random_designs(48, 100, seed=3)
+    _say("labelling 48 designs of the 100x100 bend")
+    T, G = ev.label(rhos)
+    net = pkg.fno.FNO(100, batch=16, outputs=len(sg.WAVELENGTHS), seed=0)
+    norm, history = sg.train(net, rhos, T, G, epochs=30, lr=3e-3)
+    return sg, ev, sg.inference_copy(net), norm, history, T
+
+
+def test_surrogate_loop_100x100(pkg, gpu, setup100):
+    """The loop at the workload's design size: one FDTD forward verifies
+    every proposal, the final T is the FDTD value of the final design and
+    above the start, and the verified spectrum stays below 1 (a passive
+    bend)."""
+    sg, ev, net, norm, history, T = setup100
+    assert history[-1] < history[0]
+    assert T.max() <= 1 + 1e-3
+    rho0 = sg.random_designs(1, 100, seed=7)[0]
+    out = sg.optimize(ev, rho0, steps=6, step=0.01, surrogate=(net, norm))
+    assert out["counts"]["forward"] == len(out["errors"]) == 6
+    assert out["counts"]["adjoint"] == 1 + out["fallbacks"]
+    spec = ev.transmission(out["rho"])
+    f = sg.objective_index()
+    assert abs(spec[f] - out["final"]) <= 1e-12 * abs(spec[f])
+    assert out["final"] > out["history"][0]
+    assert spec.max() <= 1 + 1e-3
