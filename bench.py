@@ -44,8 +44,8 @@ HBM_PEAK_GBS = 8000.0
 # (shape, resident, steps per launch) -> (bytes per step, the profile summary);
 # launches of another length report no traffic.
 MEASURED_TRAFFIC = {
-    (SHAPE, True, 1000): (42.72e6, "profiles/hbm_traffic_waveguide_160x96x80.md"),
-    (SHAPE, False, 1): (130.9e6, "profiles/hbm_traffic_waveguide_160x96x80.md"),
+    (SHAPE, True, 1000): (44.57e6, "profiles/hbm_traffic_waveguide_160x96x80.md"),
+    (SHAPE, False, 1): (130.2e6, "profiles/hbm_traffic_waveguide_160x96x80.md"),
 }
 
 

@@ -291,11 +291,11 @@ class Handles:
             call("wfd_stream_destroy", s)
 
 
-def release(handles):
+def release(handles, _finalizing=sys.is_finalizing):
     """``close()`` for an owner's ``__del__``: skipped while the interpreter
     is finalizing (the atexit hook has closed every live set by then) and
     silent on failure (a destructor must not raise)."""
-    if handles is None or sys.is_finalizing():
+    if handles is None or _finalizing():
         return
     try:
         handles.close()

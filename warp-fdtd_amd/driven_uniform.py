@@ -119,5 +119,7 @@ class DrivenUniform:
         self.graph = None
         self._handles.close()
 
-    def __del__(self):
-        release(getattr(self, "_handles", None))
+    def __del__(self, _release=release):
+        # (bound at definition: module globals may already be None when an
+        # object is collected during interpreter shutdown)
+        _release(getattr(self, "_handles", None))

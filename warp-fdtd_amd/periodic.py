@@ -94,5 +94,7 @@ class Periodic:
         self.graphs = {}
         self._handles.close()
 
-    def __del__(self):
-        release(getattr(self, "_handles", None))
+    def __del__(self, _release=release):
+        # (bound at definition: module globals may already be None when an
+        # object is collected during interpreter shutdown)
+        _release(getattr(self, "_handles", None))
