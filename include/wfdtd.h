@@ -157,6 +157,9 @@ typedef struct wfd_yee_targets {
                            (slot step % hist_steps, box below) */
   int hist_steps;
   int hist_box[6];      /* x0, y0, z0, sx, sy, sz */
+  int threads;          /* workgroup size: 0 (512 for one slot per thread or
+                           two steps, else 256) or 1024 with one slot per
+                           thread (a tile of 1024 - nz or 1024 - 3 nz cells) */
 } wfd_yee_targets;
 
 int wfd_yee_pass(const wfd_yee_plan* plan, const wfd_yee_targets* out, void* stream);

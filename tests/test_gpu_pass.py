@@ -74,11 +74,12 @@ def test_forward50_run_bitwise(pkg, gpu, steps):
 
 
 @pytest.mark.parametrize("shape", [(4, 2, 1), (16, 4, 1), (5, 4, 1), (2, 1, 1), (1, 1, 1),
-                                   (8, 2, 2), (4, 1, 2), (13, 2, 2), (1, 2, 2)])
+                                   (3, 1, 1, 1024), (8, 2, 2), (4, 1, 2), (13, 2, 2),
+                                   (1, 2, 2), (5, 1, 2, 1024)])
 def test_launch_shapes_and_odd_chunks_bitwise(pkg, gpu, bend, shape):
-    """Planes per workgroup x slots per thread x steps per launch, each over
-    ragged run_steps (lengths of 4k + 1..3 end with one-step passes and one
-    fused step) on the 200x200x80 bend."""
+    """Planes per workgroup x slots per thread x steps per launch (x threads
+    per workgroup), each over ragged run_steps (lengths of 4k + 1..3 end with
+    one-step passes and one fused step) on the 200x200x80 bend."""
     a = _solver(pkg, bend, False)
     b = _solver(pkg, bend, True)
     b.use_single_pass(*shape)

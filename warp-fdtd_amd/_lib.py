@@ -57,7 +57,7 @@ class YeeTargets(ctypes.Structure):
 
     _fields_ = [("e", P), ("h", P), ("psi_e", P), ("psi_h", P), ("ticket", P), ("planes", I),
                 ("slots_per_thread", I), ("steps", I), ("hist", P), ("hist_steps", I),
-                ("hist_box", I * 6)]
+                ("hist_box", I * 6), ("threads", I)]
 
 
 class InjectSource(ctypes.Structure):

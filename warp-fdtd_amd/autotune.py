@@ -185,19 +185,20 @@ def tune_resident(solver, use_cache=True, require=False):
 
 
 # steps per launch -> (x planes per workgroup, slots per thread) candidates
-PASS_PLANES = {1: (1, 2, 4, 8), 2: (2, 4, 8, 16, 32)}
-PASS_SHAPES = {1: (1, 2, 4), 2: (1, 2)}
+# (0: the library's choice from the kernel's occupancy)
+PASS_PLANES = {1: (0, 2, 4, 8), 2: (0, 4, 8, 16)}
+# (slots per thread, threads per workgroup) of each steps-per-launch
+PASS_SHAPES = {1: ((1, 512), (2, 256), (1, 1024)), 2: ((1, 512), (2, 512), (1, 1024))}
 
 
-def _pass_fits(nz, steps, slots):
-    threads = 256 if (steps == 1 and slots > 1) else 512
+def _pass_fits(nz, steps, slots, threads):
     return (nz if steps == 1 else 3 * nz) < slots * threads
 
 
-def measure_pass(solver, planes, slots, steps=1, repeats=20):
+def measure_pass(solver, planes, slots, steps=1, threads=0, repeats=20):
     """Median microseconds per step of wfd_yee_pass launches (a pair there
     and back) with the given shape; runs real steps (reset after)."""
-    solver.use_single_pass(planes, slots, steps)
+    solver.use_single_pass(planes, slots, steps, threads)
     kinds = ["p2", "skip", "p2", "skip"] if steps == 2 else ["p1", "p1"]
     try:
         def pair():
@@ -224,7 +225,7 @@ def tune_pass(solver, use_cache=True, require=False):
     """(planes, slots per thread, steps per launch) of the fastest single-pass
     shape if it beats the tuned fused step on this grid (or in any case with
     ``require``), else None."""
-    key = _key(solver) + "|single-pass-4"
+    key = _key(solver) + "|single-pass-6"
     if use_cache:
         hit = _memory.get(key)
         if hit is None:
@@ -233,8 +234,8 @@ def tune_pass(solver, use_cache=True, require=False):
             _memory[key] = hit
             return tuple(hit["shape"]) if (hit["single_pass"] or require) else None
     nz = solver.shape[2]
-    shapes = [(p, j, k) for k, slots in PASS_SHAPES.items() for j in slots
-              if _pass_fits(nz, k, j) for p in PASS_PLANES[k]]
+    shapes = [(p, j, k, t) for k, forms in PASS_SHAPES.items() for j, t in forms
+              if _pass_fits(nz, k, j, t) for p in PASS_PLANES[k]]
     if not shapes:
         if require:
             raise ValueError(f"no single-pass shape fits nz = {nz}")

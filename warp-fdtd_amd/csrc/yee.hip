@@ -484,6 +484,138 @@ __global__ __launch_bounds__(kBlock) void k_batch_h(const YeeArgs* __restrict__ 
   yee_h<XC>(table[d], blockIdx.x - first[d]);
 }
 
+// One float array of components of n cells each (E, H: 3; psi: 6; 1/eps: 1)
+// through a buffer resource: every access of a cell shares its 32-bit byte
+// offset, the component adds a uniform offset, so a lane holds one offset
+// register per cell instead of a 64-bit address per load.  The array must
+// span fewer than 2^31 bytes (checked on the host).
+struct Field {
+  Rsrc r;
+  int cb;  // bytes per component
+  __device__ __forceinline__ Field(const float* p, int comps, int n)
+      : r(buffer(p, (unsigned)comps * (unsigned)n * 4u)), cb(n * 4) {}
+  __device__ __forceinline__ float operator()(int c, int i) const { return ld(r, i * 4, c * cb); }
+  __device__ __forceinline__ void put(int c, int i, float v) const { st(r, v, i * 4, c * cb); }
+  // by byte offset: kAway reads 0 and drops the store, so a masked access
+  // needs no branch
+  __device__ __forceinline__ float at(int c, int voff) const { return ld(r, voff, c * cb); }
+  __device__ __forceinline__ void set(int c, int voff, float v) const { st(r, v, voff, c * cb); }
+};
+
+// A byte offset past the end of every Field (the host keeps the arrays of a
+// pass below 0x7ff00000 bytes, so even with a component offset added it
+// stays out of range without wrapping).
+constexpr int kAway = 0x7ffffff0;
+
+// A buffer's base address as a value the compiler cannot see through: a
+// Field built from it inside a loop is rebuilt there (a few scalar moves)
+// instead of being hoisted, which keeps nine live buffer descriptors from
+// spilling the scalar registers.
+__device__ __forceinline__ const float* opaque(const float* p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
+// The CPML coefficient rows of one cell (solver.py's coeff tables).
+struct Rows {
+  float xa, xb, xk, ya, yb, yk, za, zb, zk;
+  int my, mz;  // lane masks: all ones where the y / z row has CPML
+};
+
+// a where the mask is all ones, else b (bitwise, no lane-mask register)
+__device__ __forceinline__ float pick(int m, float a, float b) {
+  return __int_as_float((__float_as_int(a) & m) | (__float_as_int(b) & ~m));
+}
+__device__ __forceinline__ int pick(int m, int a, int b) { return (a & m) | (b & ~m); }
+
+// The six CPML partials of one cell (kernels.py:29-47, the fused step's
+// operations); ps[] becomes the updated psi where the axis has CPML.  Both
+// forms of each term are computed and one selected, so no branch splits a
+// wave whose cells straddle a CPML slab.
+__device__ __forceinline__ void partials(const Rows& r, float dx1, float dx2, float dy0,
+                                         float dy2, float dz0, float dz1, float (&ps)[6],
+                                         float& dxz, float& dxy, float& dyx, float& dyz,
+                                         float& dzy, float& dzx) {
+  const float p0 = r.xb * ps[0] + r.xa * dx2;
+  const float p1 = r.xb * ps[1] + r.xa * dx1;
+  const float p2 = r.yb * ps[2] + r.ya * dy0;
+  const float p3 = r.yb * ps[3] + r.ya * dy2;
+  const float p4 = r.zb * ps[4] + r.za * dz1;
+  const float p5 = r.zb * ps[5] + r.za * dz0;
+  const bool cx = r.xa != 0.f;  // uniform: the x row of the plane
+  dxz = cx ? r.xk * dx2 + p0 : r.xk * dx2;
+  dxy = cx ? r.xk * dx1 + p1 : r.xk * dx1;
+  dyx = pick(r.my, r.yk * dy0 + p2, r.yk * dy0);
+  dyz = pick(r.my, r.yk * dy2 + p3, r.yk * dy2);
+  dzy = pick(r.mz, r.zk * dz1 + p4, r.zk * dz1);
+  dzx = pick(r.mz, r.zk * dz0 + p5, r.zk * dz0);
+  ps[0] = cx ? p0 : ps[0];
+  ps[1] = cx ? p1 : ps[1];
+  ps[2] = pick(r.my, p2, ps[2]);
+  ps[3] = pick(r.my, p3, ps[3]);
+  ps[4] = pick(r.mz, p4, ps[4]);
+  ps[5] = pick(r.mz, p5, ps[5]);
+}
+
+// psi of one cell at byte offset v (kAway: none), where its axis has CPML
+__device__ __forceinline__ void load_psi6(const Field& psi, int v, const Rows& r,
+                                          float (&ps)[6]) {
+  const int vx = r.xa != 0.f ? v : kAway, vy = pick(r.my, v, kAway), vz = pick(r.mz, v, kAway);
+  ps[0] = psi.at(0, vx);
+  ps[1] = psi.at(1, vx);
+  ps[2] = psi.at(2, vy);
+  ps[3] = psi.at(3, vy);
+  ps[4] = psi.at(4, vz);
+  ps[5] = psi.at(5, vz);
+}
+
+__device__ __forceinline__ void store_psi6(const Field& psi, int v, const Rows& r,
+                                           const float (&ps)[6]) {
+  const int vx = r.xa != 0.f ? v : kAway, vy = pick(r.my, v, kAway), vz = pick(r.mz, v, kAway);
+  psi.set(0, vx, ps[0]);
+  psi.set(1, vx, ps[1]);
+  psi.set(2, vy, ps[2]);
+  psi.set(3, vy, ps[3]);
+  psi.set(4, vz, ps[4]);
+  psi.set(5, vz, ps[5]);
+}
+
+// The y and z CPML rows of one (y, z) offset, fixed along a march in x and so
+// loaded once per workgroup (the x row is uniform per plane).
+struct YZ {
+  float ya, yb, yk, za, zb, zk;
+  int my, mz;  // all ones where the row has CPML (ya, za nonzero)
+};
+
+__device__ __forceinline__ YZ yz_at(const float* __restrict__ c, int rows, int y, int z) {
+  YZ q;
+  q.ya = c[(rows + y) * 3];
+  q.yb = c[(rows + y) * 3 + 1];
+  q.yk = c[(rows + y) * 3 + 2];
+  q.za = c[(2 * rows + z) * 3];
+  q.zb = c[(2 * rows + z) * 3 + 1];
+  q.zk = c[(2 * rows + z) * 3 + 2];
+  q.my = q.ya != 0.f ? -1 : 0;
+  q.mz = q.za != 0.f ? -1 : 0;
+  return q;
+}
+
+__device__ __forceinline__ Rows rows_of(const float* __restrict__ c, int x, const YZ& q) {
+  Rows r;
+  r.xa = c[x * 3];
+  r.xb = c[x * 3 + 1];
+  r.xk = c[x * 3 + 2];
+  r.ya = q.ya;
+  r.yb = q.yb;
+  r.yk = q.yk;
+  r.za = q.za;
+  r.zb = q.zb;
+  r.zk = q.zk;
+  r.my = q.my;
+  r.mz = q.mz;
+  return r;
+}
+
 // ------------------------------------------------------------------------
 // Single-pass step: E and H of one step in one launch, reading the step's
 // start (E, H, psi) from one buffer set and writing its result to another
@@ -520,7 +652,7 @@ struct PassArgs {
 };
 
 template <int T, int J>
-__global__ __launch_bounds__(T) void k_yee_pass(PassArgs pa) {
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(J == 1 ? 8 : 1))) void k_yee_pass(PassArgs pa) {
   constexpr int SLOTS = J * T;
   __shared__ float lds[2][3][SLOTS];  // the new E of planes x - 1 and x
   const YeeArgs& a = pa.a;
@@ -534,14 +666,9 @@ __global__ __launch_bounds__(T) void k_yee_pass(PassArgs pa) {
     const int B = min(A + pa.W, P), Bx = min(A + pa.W + nz, P);
     const float* __restrict__ ce = a.coeff_e;
     const float* __restrict__ ch = a.coeff_h;
-    const float* __restrict__ E0 = a.e;
-    const float* __restrict__ H0 = a.h;
-    const float* __restrict__ pe0 = a.psi_e;
-    const float* __restrict__ ph0 = a.psi_h;
-    float* __restrict__ E1 = pa.e_out;
-    float* __restrict__ H1 = pa.h_out;
-    float* __restrict__ pe1 = pa.pe_out;
-    float* __restrict__ ph1 = pa.ph_out;
+    const Field E0(a.e, 3, n), H0(a.h, 3, n), IE(a.inv_eps, 1, n), pe0(a.psi_e, 6, n),
+        ph0(a.psi_h, 6, n), E1(pa.e_out, 3, n), H1(pa.h_out, 3, n), pe1(pa.pe_out, 6, n),
+        ph1(pa.ph_out, 6, n);
     const float S = a.courant;
     const bool src_on = t < a.steps && a.source_x >= x0 && a.source_x <= x1;
     const float wave_hy = src_on ? a.wave_h[2 * a.steps + t] : 0.f;
@@ -553,233 +680,227 @@ __global__ __launch_bounds__(T) void k_yee_pass(PassArgs pa) {
     int o[J], y[J], z[J];
     float hp[J][3];  // H at the plane before (the step's start)
     float ep[J][3];  // new E at the plane before
+    YZ ye[J], yh[J];
+    float sh_y[J], sh_z[J], se_y[J], se_z[J];  // source profiles at the slot's (y, z)
 #pragma unroll
     for (int j = 0; j < J; ++j) {
       o[j] = A + threadIdx.x + T * j;
       const int oc = o[j] < Bx ? o[j] : A;
       y[j] = div_small(oc, nz, a.inv_nz);
       z[j] = oc - y[j] * nz;
-      hp[j][0] = hp[j][1] = hp[j][2] = 0.f;
+      ye[j] = yz_at(ce, rows, y[j], z[j]);
+      yh[j] = yz_at(ch, rows, y[j], z[j]);
+      sh_y[j] = src_on ? a.src_h[2 * P + oc] : 0.f;
+      sh_z[j] = src_on ? a.src_h[P + oc] : 0.f;
+      se_y[j] = src_on ? a.src_e[2 * P + oc] : 0.f;
+      se_z[j] = src_on ? a.src_e[P + oc] : 0.f;
+      const int v = o[j] < Bx && x0 > 0 ? ((x0 - 1) * P + o[j]) * 4 : kAway;
+      hp[j][0] = 0.f;
+      hp[j][1] = H0.at(1, v);
+      hp[j][2] = H0.at(2, v);
       ep[j][0] = ep[j][1] = ep[j][2] = 0.f;
-      if (o[j] < Bx && x0 > 0) {
-        const int i = (x0 - 1) * P + o[j];
-        hp[j][1] = H0[n + i];
-        hp[j][2] = H0[2 * n + i];
-      }
     }
+    // Every global access of the march is unconditional, masked by an
+    // out-of-range offset: with no loads or stores under branches the
+    // compiler counts them exactly and waits for a load only, never for the
+    // stores issued after it.  (Loading plane x + 1 during the arithmetic of
+    // plane x measured no faster: more registers, fewer workgroups per CU.)
+    struct In {
+      float hn[3], yx, yz, zx, zy, ex[3], ie, ps[6], psh[6];
+    };
+    // the step's start at plane xl (for its new E) and the H psi of plane
+    // xl - 1 (for its new H); zeros where the march does not reach
+    auto load = [&](int xl, In (&q)[J]) {
+      const bool e_on = xl < nx && xl <= x1, h_on = xl > x0 && xl <= x1;  // uniform
+      const int xe = e_on ? xl : 0, xg = h_on ? xl - 1 : 0;
+      const float xa = e_on ? ce[xe * 3] : 0.f, hxa = h_on ? ch[xg * 3] : 0.f;
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const bool on = e_on && o[j] < Bx;
+        const int v = on ? (xl * P + o[j]) * 4 : kAway;
+        const int vy = on && y[j] > 0 ? v - 4 * nz : kAway;
+        const int vz = on && z[j] > 0 ? v - 4 : kAway;
+        for (int c = 0; c < 3; ++c) q[j].hn[c] = H0.at(c, v);
+        q[j].yx = H0.at(0, vy);
+        q[j].yz = H0.at(2, vy);
+        q[j].zx = H0.at(0, vz);
+        q[j].zy = H0.at(1, vz);
+        for (int c = 0; c < 3; ++c) q[j].ex[c] = E0.at(c, v);
+        q[j].ie = IE.at(0, v);
+        const int vx_ = xa != 0.f ? v : kAway, vy_ = ye[j].ya != 0.f ? v : kAway,
+                  vz_ = ye[j].za != 0.f ? v : kAway;
+        q[j].ps[0] = pe0.at(0, vx_);
+        q[j].ps[1] = pe0.at(1, vx_);
+        q[j].ps[2] = pe0.at(2, vy_);
+        q[j].ps[3] = pe0.at(3, vy_);
+        q[j].ps[4] = pe0.at(4, vz_);
+        q[j].ps[5] = pe0.at(5, vz_);
+        const int w = h_on && o[j] < B ? ((xl - 1) * P + o[j]) * 4 : kAway;
+        const int wx = hxa != 0.f ? w : kAway, wy = yh[j].ya != 0.f ? w : kAway,
+                  wz = yh[j].za != 0.f ? w : kAway;
+        q[j].psh[0] = ph0.at(0, wx);
+        q[j].psh[1] = ph0.at(1, wx);
+        q[j].psh[2] = ph0.at(2, wy);
+        q[j].psh[3] = ph0.at(3, wy);
+        q[j].psh[4] = ph0.at(4, wz);
+        q[j].psh[5] = ph0.at(5, wz);
+      }
+    };
 #pragma unroll 1
     for (int x = x0; x <= x1; ++x) {
       float* __restrict__ cur = &lds[x & 1][0][0];
       const float* __restrict__ prev = &lds[(x + 1) & 1][0][0];
-      // the new E of plane x over the tile and its y + 1 rows
-      float en[J][3], hn[J][3];
-      if (x < nx) {
-        const float xa = ce[x * 3], xb = ce[x * 3 + 1], xk = ce[x * 3 + 2];
-        // loads first (every slot), then the arithmetic
-        float hx[J], yx[J], yz[J], zx[J], zy[J], ex[J], ey[J], ez[J], ie[J], ps[J][6];
+      const int xh = x - 1;
+      const bool e_plane = x < nx, h_plane = x > x0;  // uniform
+      // x rows read unconditionally at a clamped plane (scalar loads)
+      const int xe = e_plane ? x : nx - 1, xhc = h_plane ? xh : x0;
+      const float xa = e_plane ? ce[xe * 3] : 0.f, xb = ce[xe * 3 + 1], xk = ce[xe * 3 + 2];
+      const float hxa = h_plane ? ch[xhc * 3] : 0.f, hxb = ch[xhc * 3 + 1],
+                  hxk = ch[xhc * 3 + 2];
+      In in[J];
+      load(x, in);
+      float hn[J][3], yx[J], yz[J], zx[J], zy[J], ex[J][3], ie[J], ps[J][6], psh[J][6];
 #pragma unroll
-        for (int j = 0; j < J; ++j) {
-          hn[j][0] = hn[j][1] = hn[j][2] = 0.f;
-          if (o[j] < Bx) {
-            const int i = x * P + o[j];
-            hn[j][0] = H0[i];
-            hn[j][1] = H0[n + i];
-            hn[j][2] = H0[2 * n + i];
-            yx[j] = y[j] > 0 ? H0[i - nz] : 0.f;
-            yz[j] = y[j] > 0 ? H0[2 * n + i - nz] : 0.f;
-            zx[j] = z[j] > 0 ? H0[i - 1] : 0.f;
-            zy[j] = z[j] > 0 ? H0[n + i - 1] : 0.f;
-            ex[j] = E0[i];
-            ey[j] = E0[n + i];
-            ez[j] = E0[2 * n + i];
-            ie[j] = a.inv_eps[i];
-            const float ya = ce[(rows + y[j]) * 3], za = ce[(2 * rows + z[j]) * 3];
-            ps[j][0] = xa != 0.f ? pe0[i] : 0.f;
-            ps[j][1] = xa != 0.f ? pe0[n + i] : 0.f;
-            ps[j][2] = ya != 0.f ? pe0[2 * n + i] : 0.f;
-            ps[j][3] = ya != 0.f ? pe0[3 * n + i] : 0.f;
-            ps[j][4] = za != 0.f ? pe0[4 * n + i] : 0.f;
-            ps[j][5] = za != 0.f ? pe0[5 * n + i] : 0.f;
-          }
-          hx[j] = hn[j][0];
+      for (int j = 0; j < J; ++j) {
+        for (int c = 0; c < 3; ++c) {
+          hn[j][c] = in[j].hn[c];
+          ex[j][c] = in[j].ex[c];
         }
+        yx[j] = in[j].yx;
+        yz[j] = in[j].yz;
+        zx[j] = in[j].zx;
+        zy[j] = in[j].zy;
+        ie[j] = in[j].ie;
+        for (int c = 0; c < 6; ++c) {
+          ps[j][c] = in[j].ps[c];
+          psh[j][c] = in[j].psh[c];
+        }
+      }
+      // the new E of plane x over the tile and its y + 1 rows (zero past Bx)
+      float en[J][3];
+#pragma unroll
+      for (int j = 0; j < J; ++j) en[j][0] = en[j][1] = en[j][2] = 0.f;
+      if (e_plane) {
 #pragma unroll
         for (int j = 0; j < J; ++j) {
-          en[j][0] = en[j][1] = en[j][2] = 0.f;
-          if (o[j] < Bx) {
-            const int i = x * P + o[j];
-            const bool keep = x < x1 && o[j] < B;  // owned: stored
-            const float ya = ce[(rows + y[j]) * 3], yb = ce[(rows + y[j]) * 3 + 1],
-                        yk = ce[(rows + y[j]) * 3 + 2];
-            const float za = ce[(2 * rows + z[j]) * 3], zb = ce[(2 * rows + z[j]) * 3 + 1],
-                        zk = ce[(2 * rows + z[j]) * 3 + 2];
-            const float dx1 = hn[j][1] - hp[j][1], dx2 = hn[j][2] - hp[j][2];
-            const float dy0 = hx[j] - yx[j], dy2 = hn[j][2] - yz[j];
-            const float dz0 = hx[j] - zx[j], dz1 = hn[j][1] - zy[j];
-            float dxz, dxy, dyx, dyz, dzy, dzx;
-            if (xa != 0.f) {
-              const float p0 = xb * ps[j][0] + xa * dx2;
-              const float p1 = xb * ps[j][1] + xa * dx1;
-              if (keep) {
-                pe1[i] = p0;
-                pe1[n + i] = p1;
-              }
-              dxz = xk * dx2 + p0;
-              dxy = xk * dx1 + p1;
-            } else {
-              dxz = xk * dx2;
-              dxy = xk * dx1;
-            }
-            if (ya != 0.f) {
-              const float p2 = yb * ps[j][2] + ya * dy0;
-              const float p3 = yb * ps[j][3] + ya * dy2;
-              if (keep) {
-                pe1[2 * n + i] = p2;
-                pe1[3 * n + i] = p3;
-              }
-              dyx = yk * dy0 + p2;
-              dyz = yk * dy2 + p3;
-            } else {
-              dyx = yk * dy0;
-              dyz = yk * dy2;
-            }
-            if (za != 0.f) {
-              const float p4 = zb * ps[j][4] + za * dz1;
-              const float p5 = zb * ps[j][5] + za * dz0;
-              if (keep) {
-                pe1[4 * n + i] = p4;
-                pe1[5 * n + i] = p5;
-              }
-              dzy = zk * dz1 + p4;
-              dzx = zk * dz0 + p5;
-            } else {
-              dzy = zk * dz1;
-              dzx = zk * dz0;
-            }
-            float vx = ex[j] + (S * (dyz - dzy)) * ie[j];
-            float vy = ey[j] + (S * (dzx - dxz)) * ie[j];
-            float vz = ez[j] + (S * (dxy - dyx)) * ie[j];
-            if (src_on && x == a.source_x) {
-              vy += ((a.src_h[2 * P + o[j]] * wave_hy) * S) * ie[j];
-              vz -= ((a.src_h[P + o[j]] * wave_hz) * S) * ie[j];
-            }
-            en[j][0] = vx;
-            en[j][1] = vy;
-            en[j][2] = vz;
-            if (keep) {
-              E1[i] = vx;
-              E1[n + i] = vy;
-              E1[2 * n + i] = vz;
-            }
+          const int v = x < x1 && o[j] < B ? (x * P + o[j]) * 4 : kAway;  // owned: stored
+          const float ya = ye[j].ya, yb = ye[j].yb, yk = ye[j].yk;
+          const float za = ye[j].za, zb = ye[j].zb, zk = ye[j].zk;
+          const float dx1 = hn[j][1] - hp[j][1], dx2 = hn[j][2] - hp[j][2];
+          const float dy0 = hn[j][0] - yx[j], dy2 = hn[j][2] - yz[j];
+          const float dz0 = hn[j][0] - zx[j], dz1 = hn[j][1] - zy[j];
+          // both forms of each CPML term, then a select (no branch holds a
+          // load or store)
+          const float p0 = xb * ps[j][0] + xa * dx2;
+          const float p1 = xb * ps[j][1] + xa * dx1;
+          const int vx_ = xa != 0.f ? v : kAway;
+          pe1.set(0, vx_, p0);
+          pe1.set(1, vx_, p1);
+          const float dxz = xa != 0.f ? xk * dx2 + p0 : xk * dx2;
+          const float dxy = xa != 0.f ? xk * dx1 + p1 : xk * dx1;
+          const float p2 = yb * ps[j][2] + ya * dy0;
+          const float p3 = yb * ps[j][3] + ya * dy2;
+          const int vy_ = ya != 0.f ? v : kAway;
+          pe1.set(2, vy_, p2);
+          pe1.set(3, vy_, p3);
+          const float dyx = ya != 0.f ? yk * dy0 + p2 : yk * dy0;
+          const float dyz = ya != 0.f ? yk * dy2 + p3 : yk * dy2;
+          const float p4 = zb * ps[j][4] + za * dz1;
+          const float p5 = zb * ps[j][5] + za * dz0;
+          const int vz_ = za != 0.f ? v : kAway;
+          pe1.set(4, vz_, p4);
+          pe1.set(5, vz_, p5);
+          const float dzy = za != 0.f ? zk * dz1 + p4 : zk * dz1;
+          const float dzx = za != 0.f ? zk * dz0 + p5 : zk * dz0;
+          float vx = ex[j][0] + (S * (dyz - dzy)) * ie[j];
+          float vy = ex[j][1] + (S * (dzx - dxz)) * ie[j];
+          float vz = ex[j][2] + (S * (dxy - dyx)) * ie[j];
+          if (src_on && x == a.source_x) {
+            vy += ((sh_y[j] * wave_hy) * S) * ie[j];
+            vz -= ((sh_z[j] * wave_hz) * S) * ie[j];
           }
+          en[j][0] = vx;
+          en[j][1] = vy;
+          en[j][2] = vz;
+          E1.set(0, v, vx);
+          E1.set(1, v, vy);
+          E1.set(2, v, vz);
           const int s = threadIdx.x + T * j;
-          cur[s] = en[j][0];
-          cur[SLOTS + s] = en[j][1];
-          cur[2 * SLOTS + s] = en[j][2];
+          cur[s] = vx;
+          cur[SLOTS + s] = vy;
+          cur[2 * SLOTS + s] = vz;
         }
-      } else {
-#pragma unroll
-        for (int j = 0; j < J; ++j)
-          en[j][0] = en[j][1] = en[j][2] = hn[j][0] = hn[j][1] = hn[j][2] = 0.f;
       }
       // the new H of plane x - 1 over the tile
-      if (x > x0) {
-        const int xh = x - 1;
-        const float xa = ch[xh * 3], xb = ch[xh * 3 + 1], xk = ch[xh * 3 + 2];
-        float ps[J][6];
+      if (h_plane) {
 #pragma unroll
         for (int j = 0; j < J; ++j) {
-          if (o[j] < B) {
+          const bool own = o[j] < B;
+          const int w = own ? (xh * P + o[j]) * 4 : kAway;
+          const int s = threadIdx.x + T * j;
+          // (the clamps only touch slots past the tile, whose results are dropped)
+          const int sy = min(s + nz, SLOTS - 1), sz = min(s + 1, SLOTS - 1);
+          const bool up_y = y[j] + 1 < ny, up_z = z[j] + 1 < nz;
+          const float qx = up_y ? prev[sy] : 0.f;
+          const float qz = up_y ? prev[2 * SLOTS + sy] : 0.f;
+          const float rx = up_z ? prev[sz] : 0.f;
+          const float ry = up_z ? prev[SLOTS + sz] : 0.f;
+          const float dx1 = en[j][1] - ep[j][1], dx2 = en[j][2] - ep[j][2];
+          const float dy0 = qx - ep[j][0], dy2 = qz - ep[j][2];
+          const float dz0 = rx - ep[j][0], dz1 = ry - ep[j][1];
+          const float ya = yh[j].ya, yb = yh[j].yb, yk = yh[j].yk;
+          const float za = yh[j].za, zb = yh[j].zb, zk = yh[j].zk;
+          const float p0 = hxb * psh[j][0] + hxa * dx2;
+          const float p1 = hxb * psh[j][1] + hxa * dx1;
+          const int wx = hxa != 0.f ? w : kAway;
+          ph1.set(0, wx, p0);
+          ph1.set(1, wx, p1);
+          const float dxz = hxa != 0.f ? hxk * dx2 + p0 : hxk * dx2;
+          const float dxy = hxa != 0.f ? hxk * dx1 + p1 : hxk * dx1;
+          const float p2 = yb * psh[j][2] + ya * dy0;
+          const float p3 = yb * psh[j][3] + ya * dy2;
+          const int wy = ya != 0.f ? w : kAway;
+          ph1.set(2, wy, p2);
+          ph1.set(3, wy, p3);
+          const float dyx = ya != 0.f ? yk * dy0 + p2 : yk * dy0;
+          const float dyz = ya != 0.f ? yk * dy2 + p3 : yk * dy2;
+          const float p4 = zb * psh[j][4] + za * dz1;
+          const float p5 = zb * psh[j][5] + za * dz0;
+          const int wz = za != 0.f ? w : kAway;
+          ph1.set(4, wz, p4);
+          ph1.set(5, wz, p5);
+          const float dzy = za != 0.f ? zk * dz1 + p4 : zk * dz1;
+          const float dzx = za != 0.f ? zk * dz0 + p5 : zk * dz0;
+          const float hx = hp[j][0], hy = hp[j][1], hz = hp[j][2];
+          if (own && save_t && xh >= a.hbox[0] && xh < a.hbox[1] && y[j] >= a.hbox[2] &&
+              y[j] < a.hbox[3] && z[j] >= a.hbox[4] && z[j] < a.hbox[5]) {
             const int i = xh * P + o[j];
-            const float ya = ch[(rows + y[j]) * 3], za = ch[(2 * rows + z[j]) * 3];
-            ps[j][0] = xa != 0.f ? ph0[i] : 0.f;
-            ps[j][1] = xa != 0.f ? ph0[n + i] : 0.f;
-            ps[j][2] = ya != 0.f ? ph0[2 * n + i] : 0.f;
-            ps[j][3] = ya != 0.f ? ph0[3 * n + i] : 0.f;
-            ps[j][4] = za != 0.f ? ph0[4 * n + i] : 0.f;
-            ps[j][5] = za != 0.f ? ph0[5 * n + i] : 0.f;
+            a.hprev[i] = hx;
+            a.hprev[n + i] = hy;
+            a.hprev[2 * n + i] = hz;
           }
-        }
-#pragma unroll
-        for (int j = 0; j < J; ++j) {
-          if (o[j] < B) {
-            const int i = xh * P + o[j];
-            const int s = threadIdx.x + T * j;
-            const bool up_y = y[j] + 1 < ny, up_z = z[j] + 1 < nz;
-            const float qx = up_y ? prev[s + nz] : 0.f;
-            const float qz = up_y ? prev[2 * SLOTS + s + nz] : 0.f;
-            const float rx = up_z ? prev[s + 1] : 0.f;
-            const float ry = up_z ? prev[SLOTS + s + 1] : 0.f;
-            const float dx1 = en[j][1] - ep[j][1], dx2 = en[j][2] - ep[j][2];
-            const float dy0 = qx - ep[j][0], dy2 = qz - ep[j][2];
-            const float dz0 = rx - ep[j][0], dz1 = ry - ep[j][1];
-            const float ya = ch[(rows + y[j]) * 3], yb = ch[(rows + y[j]) * 3 + 1],
-                        yk = ch[(rows + y[j]) * 3 + 2];
-            const float za = ch[(2 * rows + z[j]) * 3], zb = ch[(2 * rows + z[j]) * 3 + 1],
-                        zk = ch[(2 * rows + z[j]) * 3 + 2];
-            float dxz, dxy, dyx, dyz, dzy, dzx;
-            if (xa != 0.f) {
-              const float p0 = xb * ps[j][0] + xa * dx2;
-              const float p1 = xb * ps[j][1] + xa * dx1;
-              ph1[i] = p0;
-              ph1[n + i] = p1;
-              dxz = xk * dx2 + p0;
-              dxy = xk * dx1 + p1;
-            } else {
-              dxz = xk * dx2;
-              dxy = xk * dx1;
-            }
-            if (ya != 0.f) {
-              const float p2 = yb * ps[j][2] + ya * dy0;
-              const float p3 = yb * ps[j][3] + ya * dy2;
-              ph1[2 * n + i] = p2;
-              ph1[3 * n + i] = p3;
-              dyx = yk * dy0 + p2;
-              dyz = yk * dy2 + p3;
-            } else {
-              dyx = yk * dy0;
-              dyz = yk * dy2;
-            }
-            if (za != 0.f) {
-              const float p4 = zb * ps[j][4] + za * dz1;
-              const float p5 = zb * ps[j][5] + za * dz0;
-              ph1[4 * n + i] = p4;
-              ph1[5 * n + i] = p5;
-              dzy = zk * dz1 + p4;
-              dzx = zk * dz0 + p5;
-            } else {
-              dzy = zk * dz1;
-              dzx = zk * dz0;
-            }
-            const float hx = hp[j][0], hy = hp[j][1], hz = hp[j][2];
-            if (save_t && xh >= a.hbox[0] && xh < a.hbox[1] && y[j] >= a.hbox[2] &&
-                y[j] < a.hbox[3] && z[j] >= a.hbox[4] && z[j] < a.hbox[5]) {
-              a.hprev[i] = hx;
-              a.hprev[n + i] = hy;
-              a.hprev[2 * n + i] = hz;
-            }
-            float vx = hx - S * (dyz - dzy);
-            float vy = hy - S * (dzx - dxz);
-            float vz = hz - S * (dxy - dyx);
-            if (src_on && xh == a.source_x) {
-              vy -= (a.src_e[2 * P + o[j]] * wave_ey) * S;
-              vz += (a.src_e[P + o[j]] * wave_ez) * S;
-            }
-            H1[i] = vx;
-            H1[n + i] = vy;
-            H1[2 * n + i] = vz;
-            if (a.nstage)
-              stage_cell(a.st, a.nstage, t % a.slots, xh, y[j], z[j], ep[j][0], ep[j][1],
-                         ep[j][2], vx, vy, vz, hx, hy, hz);
+          float vx = hx - S * (dyz - dzy);
+          float vy = hy - S * (dzx - dxz);
+          float vz = hz - S * (dxy - dyx);
+          if (src_on && xh == a.source_x) {
+            vy -= (se_y[j] * wave_ey) * S;
+            vz += (se_z[j] * wave_ez) * S;
           }
+          H1.set(0, w, vx);
+          H1.set(1, w, vy);
+          H1.set(2, w, vz);
+          if (a.nstage && own)
+            stage_cell(a.st, a.nstage, t % a.slots, xh, y[j], z[j], ep[j][0], ep[j][1],
+                       ep[j][2], vx, vy, vz, hx, hy, hz);
         }
       }
 #pragma unroll
-      for (int j = 0; j < J; ++j)
+      for (int j = 0; j < J; ++j) {
         for (int c = 0; c < 3; ++c) {
           hp[j][c] = hn[j][c];
           ep[j][c] = en[j][c];
         }
+      }
       __syncthreads();
     }
   }
@@ -809,91 +930,6 @@ __global__ __launch_bounds__(T) void k_yee_pass(PassArgs pa) {
 // needed along x stays in the registers of the thread that holds its
 // offset.  Reads E, H, 1/eps once and writes E, H once per two
 // cell-updates; psi is read and written once per two steps as well.
-struct Rows {
-  float xa, xb, xk, ya, yb, yk, za, zb, zk;
-};
-
-__device__ __forceinline__ Rows rows_at(const float* __restrict__ c, int rows, int x, int y,
-                                        int z) {
-  Rows r;
-  r.xa = c[x * 3];
-  r.xb = c[x * 3 + 1];
-  r.xk = c[x * 3 + 2];
-  r.ya = c[(rows + y) * 3];
-  r.yb = c[(rows + y) * 3 + 1];
-  r.yk = c[(rows + y) * 3 + 2];
-  r.za = c[(2 * rows + z) * 3];
-  r.zb = c[(2 * rows + z) * 3 + 1];
-  r.zk = c[(2 * rows + z) * 3 + 2];
-  return r;
-}
-
-// The six CPML partials of one cell (kernels.py:29-47, the fused step's
-// operations); ps[] becomes the updated psi where the axis has CPML.
-__device__ __forceinline__ void partials(const Rows& r, float dx1, float dx2, float dy0,
-                                         float dy2, float dz0, float dz1, float (&ps)[6],
-                                         float& dxz, float& dxy, float& dyx, float& dyz,
-                                         float& dzy, float& dzx) {
-  if (r.xa != 0.f) {
-    const float p0 = r.xb * ps[0] + r.xa * dx2;
-    const float p1 = r.xb * ps[1] + r.xa * dx1;
-    ps[0] = p0;
-    ps[1] = p1;
-    dxz = r.xk * dx2 + p0;
-    dxy = r.xk * dx1 + p1;
-  } else {
-    dxz = r.xk * dx2;
-    dxy = r.xk * dx1;
-  }
-  if (r.ya != 0.f) {
-    const float p2 = r.yb * ps[2] + r.ya * dy0;
-    const float p3 = r.yb * ps[3] + r.ya * dy2;
-    ps[2] = p2;
-    ps[3] = p3;
-    dyx = r.yk * dy0 + p2;
-    dyz = r.yk * dy2 + p3;
-  } else {
-    dyx = r.yk * dy0;
-    dyz = r.yk * dy2;
-  }
-  if (r.za != 0.f) {
-    const float p4 = r.zb * ps[4] + r.za * dz1;
-    const float p5 = r.zb * ps[5] + r.za * dz0;
-    ps[4] = p4;
-    ps[5] = p5;
-    dzy = r.zk * dz1 + p4;
-    dzx = r.zk * dz0 + p5;
-  } else {
-    dzy = r.zk * dz1;
-    dzx = r.zk * dz0;
-  }
-}
-
-__device__ __forceinline__ void load_psi6(const float* __restrict__ psi, int n, int i,
-                                          const Rows& r, float (&ps)[6]) {
-  ps[0] = r.xa != 0.f ? psi[i] : 0.f;
-  ps[1] = r.xa != 0.f ? psi[n + i] : 0.f;
-  ps[2] = r.ya != 0.f ? psi[2 * n + i] : 0.f;
-  ps[3] = r.ya != 0.f ? psi[3 * n + i] : 0.f;
-  ps[4] = r.za != 0.f ? psi[4 * n + i] : 0.f;
-  ps[5] = r.za != 0.f ? psi[5 * n + i] : 0.f;
-}
-
-__device__ __forceinline__ void store_psi6(float* __restrict__ psi, int n, int i, const Rows& r,
-                                           const float (&ps)[6]) {
-  if (r.xa != 0.f) {
-    psi[i] = ps[0];
-    psi[n + i] = ps[1];
-  }
-  if (r.ya != 0.f) {
-    psi[2 * n + i] = ps[2];
-    psi[3 * n + i] = ps[3];
-  }
-  if (r.za != 0.f) {
-    psi[4 * n + i] = ps[4];
-    psi[5 * n + i] = ps[5];
-  }
-}
 
 // The step's-start values one new-E computation of the two-step pass reads
 // at a cell: H and its y - 1 / z - 1 neighbours, E, 1/eps and the E psi.
@@ -901,32 +937,38 @@ struct PlaneIn {
   float h[3], yx, yz, zx, zy, e[3], ie, ps[6];
 };
 
-__device__ __forceinline__ void load_plane(const YeeArgs& a, int x, int o, int y, int z,
-                                           bool on, PlaneIn& q) {
-  const float* __restrict__ H0 = a.h;
-  const float* __restrict__ E0 = a.e;
-  const int n = a.n, nz = a.nz;
-  q.h[0] = q.h[1] = q.h[2] = q.yx = q.yz = q.zx = q.zy = 0.f;
-  q.e[0] = q.e[1] = q.e[2] = q.ie = 0.f;
-  for (int c = 0; c < 6; ++c) q.ps[c] = 0.f;
-  if (!on) return;
-  const int i = x * a.P + o;
-  q.h[0] = H0[i];
-  q.h[1] = H0[n + i];
-  q.h[2] = H0[2 * n + i];
-  if (y > 0) {
-    q.yx = H0[i - nz];
-    q.yz = H0[2 * n + i - nz];
-  }
-  if (z > 0) {
-    q.zx = H0[i - 1];
-    q.zy = H0[n + i - 1];
-  }
-  q.e[0] = E0[i];
-  q.e[1] = E0[n + i];
-  q.e[2] = E0[2 * n + i];
-  q.ie = a.inv_eps[i];
-  load_psi6(a.psi_e, n, i, rows_at(a.coeff_e, a.rows, x, y, z), q.ps);
+// The arrays of a single pass: the step's start and its result.
+struct PassFields {
+  Field e0, h0, ie, pe0, ph0, e1, h1, pe1, ph1;
+  __device__ __forceinline__ PassFields(const PassArgs& pa)
+      : e0(opaque(pa.a.e), 3, pa.a.n),
+        h0(opaque(pa.a.h), 3, pa.a.n),
+        ie(opaque(pa.a.inv_eps), 1, pa.a.n),
+        pe0(opaque(pa.a.psi_e), 6, pa.a.n),
+        ph0(opaque(pa.a.psi_h), 6, pa.a.n),
+        e1(opaque(pa.e_out), 3, pa.a.n),
+        h1(opaque(pa.h_out), 3, pa.a.n),
+        pe1(opaque(pa.pe_out), 6, pa.a.n),
+        ph1(opaque(pa.ph_out), 6, pa.a.n) {}
+};
+
+// Plane loads at byte offset v of the cell (kAway: zeros), branch-free.
+__device__ __forceinline__ void load_plane(const YeeArgs& a, const PassFields& f, int x, int v,
+                                           int y, int z, const YZ& ye, PlaneIn& q) {
+  const int vy = v != kAway && y > 0 ? v - 4 * a.nz : kAway;
+  const int vz = v != kAway && z > 0 ? v - 4 : kAway;
+  q.h[0] = f.h0.at(0, v);
+  q.h[1] = f.h0.at(1, v);
+  q.h[2] = f.h0.at(2, v);
+  q.yx = f.h0.at(0, vy);
+  q.yz = f.h0.at(2, vy);
+  q.zx = f.h0.at(0, vz);
+  q.zy = f.h0.at(1, vz);
+  q.e[0] = f.e0.at(0, v);
+  q.e[1] = f.e0.at(1, v);
+  q.e[2] = f.e0.at(2, v);
+  q.ie = f.ie.at(0, v);
+  load_psi6(f.pe0, v, rows_of(a.coeff_e, x, ye), q.ps);
 }
 
 // wfd_record_curl's value (adjoint.py:17-43) of one cell at the start of step
@@ -958,7 +1000,7 @@ __global__ __launch_bounds__(T) void k_yee_pass2(PassArgs pa) {
   const int logical = (blockIdx.x & 7) * pa.per + (blockIdx.x >> 3);
   const int t = a.counters[0];
   if (logical < pa.total) {
-    const int nx = a.nx, ny = a.ny, nz = a.nz, n = a.n, P = a.P, rows = a.rows;
+    const int nx = a.nx, ny = a.ny, nz = a.nz, P = a.P, rows = a.rows;
     const int tile = logical % pa.tiles, chunk = logical / pa.tiles;
     const int A = tile * pa.W;
     const int x0 = chunk * pa.xc, x1 = min(x0 + pa.xc, nx);
@@ -967,12 +1009,12 @@ __global__ __launch_bounds__(T) void k_yee_pass2(PassArgs pa) {
     const int lo = A - nz;                                   // offset of slot 0
     const float* __restrict__ ce = a.coeff_e;
     const float* __restrict__ ch = a.coeff_h;
-    const float* __restrict__ E0 = a.e;
-    const float* __restrict__ H0 = a.h;
+    const PassFields f(pa);
     const float S = a.courant;
     // sources of the two steps
     const bool src1 = t < a.steps && a.source_x >= x0 - 1 && a.source_x <= x1 + 1;
     const bool src2 = t + 1 < a.steps && a.source_x >= x0 - 1 && a.source_x <= x1 + 1;
+    const bool src = src1 || src2;
     const float wh1y = src1 ? a.wave_h[2 * a.steps + t] : 0.f;
     const float wh1z = src1 ? a.wave_h[a.steps + t] : 0.f;
     const float we1y = src1 ? a.wave_e[2 * a.steps + t] : 0.f;
@@ -985,100 +1027,120 @@ __global__ __launch_bounds__(T) void k_yee_pass2(PassArgs pa) {
     const int xs = max(x0 - 1, 0);
     int o[J], y[J], z[J];
     float hp0[J][3], e1p[J][3], pse1[J][6], ie1[J], h1p[J][3], psh1[J][6], e2p[J][3];
+    float sh_y[J], sh_z[J], se_y[J], se_z[J];  // source profiles at the slot's (y, z)
+    YZ ye[J], yh[J];
 #pragma unroll
     for (int j = 0; j < J; ++j) {
       o[j] = lo + (int)threadIdx.x + T * j;
       const int oc = (o[j] >= 0 && o[j] < Be) ? o[j] : max(A, 0);
       y[j] = div_small(oc, nz, a.inv_nz);
       z[j] = oc - y[j] * nz;
+      ye[j] = yz_at(ce, rows, y[j], z[j]);
+      yh[j] = yz_at(ch, rows, y[j], z[j]);
+      sh_y[j] = src ? a.src_h[2 * P + oc] : 0.f;
+      sh_z[j] = src ? a.src_h[P + oc] : 0.f;
+      se_y[j] = src ? a.src_e[2 * P + oc] : 0.f;
+      se_z[j] = src ? a.src_e[P + oc] : 0.f;
       for (int c = 0; c < 3; ++c) hp0[j][c] = e1p[j][c] = h1p[j][c] = e2p[j][c] = 0.f;
       for (int c = 0; c < 6; ++c) pse1[j][c] = psh1[j][c] = 0.f;
       ie1[j] = 0.f;
-      if (o[j] >= 0 && o[j] < Be && xs > 0) {
-        const int i = (xs - 1) * P + o[j];
-        hp0[j][1] = H0[n + i];
-        hp0[j][2] = H0[2 * n + i];
-      }
+      const int v = o[j] >= 0 && o[j] < Be && xs > 0 ? ((xs - 1) * P + o[j]) * 4 : kAway;
+      hp0[j][1] = f.h0.at(1, v);
+      hp0[j][2] = f.h0.at(2, v);
     }
+    // As in the one-step pass, every global load and store of the march is
+    // unconditional, masked by an out-of-range offset, and the CPML terms are
+    // selected rather than branched on.
 #pragma unroll 1
     for (int x = xs; x <= x1 + 1; ++x) {
-      float en[J][3], hn[J][3], pse[J][6], iex[J], h1n[J][3], psh[J][6], e2n[J][3];
-      // every load of the plane first (loading the next plane one iteration
-      // ahead measured slower: more registers, fewer waves)
-      PlaneIn in[J];
+      // The per-slot values below are loop-invariant; hiding that from the
+      // compiler keeps it from hoisting the lane masks derived from them
+      // (each a 64-bit scalar register pair) out of the march, which ran it
+      // out of scalar registers into spills.
 #pragma unroll
-      for (int j = 0; j < J; ++j)
-        load_plane(a, x, o[j], y[j], z[j], x < nx && o[j] >= 0 && o[j] < Be, in[j]);
+      for (int j = 0; j < J; ++j) {
+        asm volatile("" : "+v"(o[j]), "+v"(y[j]), "+v"(z[j]));
+        asm volatile("" : "+v"(ye[j].my), "+v"(ye[j].mz), "+v"(yh[j].my), "+v"(yh[j].mz));
+      }
+      const int xh = x - 1, xq = x - 2;
+      // uniform: which of the four updates this plane of the march makes
+      const bool do_a = x < nx;
+      const bool do_b = xh >= x0 - 1 && xh >= 0 && xh < nx;
+      const bool do_c = xh >= x0 && xh <= x1 && xh < nx;
+      const bool do_d = xq >= x0 && xq < x1;
+      const int cx = do_a ? x : 0, cxh = do_b || do_c ? xh : 0, cxq = do_d ? xq : 0;
+      float en[J][3], hn[J][3], pse[J][6], iex[J], h1n[J][3], psh[J][6], e2n[J][3];
+      PlaneIn in[J];
+      PassFields f(pa);  // rebuilt per plane (see opaque)
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const bool on = do_a && o[j] >= 0 && o[j] < Be;
+        load_plane(a, f, cx, on ? (x * P + o[j]) * 4 : kAway, y[j], z[j], ye[j], in[j]);
+        // the H psi of plane x - 1, for (b)
+        const bool hon = do_b && o[j] >= 0 && o[j] < Bh;
+        load_psi6(f.ph0, hon ? (xh * P + o[j]) * 4 : kAway, rows_of(ch, cxh, yh[j]), psh[j]);
+      }
       // (a) E1 of plane x over [A - nz, B + 2 nz)
 #pragma unroll
       for (int j = 0; j < J; ++j) {
-        for (int c = 0; c < 3; ++c) en[j][c] = hn[j][c] = 0.f;
-        for (int c = 0; c < 6; ++c) pse[j][c] = 0.f;
-        iex[j] = 0.f;
-        if (x < nx && o[j] >= 0 && o[j] < Be) {
-          const Rows r = rows_at(ce, rows, x, y[j], z[j]);
-          for (int c = 0; c < 3; ++c) hn[j][c] = in[j].h[c];
-          for (int c = 0; c < 6; ++c) pse[j][c] = in[j].ps[c];
-          iex[j] = in[j].ie;
-          const float dx1 = hn[j][1] - hp0[j][1], dx2 = hn[j][2] - hp0[j][2];
-          const float dy0 = hn[j][0] - in[j].yx, dy2 = hn[j][2] - in[j].yz;
-          const float dz0 = hn[j][0] - in[j].zx, dz1 = hn[j][1] - in[j].zy;
-          if (pa.hist && x >= x0 && x < x1 && o[j] >= A && o[j] < B)
-            record_curl_at(pa, t, x, y[j], z[j], dx1, dx2, dy0, dy2, dz0, dz1);
-          float dxz, dxy, dyx, dyz, dzy, dzx;
-          partials(r, dx1, dx2, dy0, dy2, dz0, dz1, pse[j], dxz, dxy, dyx, dyz, dzy, dzx);
-          float vx = in[j].e[0] + (S * (dyz - dzy)) * iex[j];
-          float vy = in[j].e[1] + (S * (dzx - dxz)) * iex[j];
-          float vz = in[j].e[2] + (S * (dxy - dyx)) * iex[j];
-          if (src1 && x == a.source_x) {
-            vy += ((a.src_h[2 * P + o[j]] * wh1y) * S) * iex[j];
-            vz -= ((a.src_h[P + o[j]] * wh1z) * S) * iex[j];
-          }
-          en[j][0] = vx;
-          en[j][1] = vy;
-          en[j][2] = vz;
+        const bool on = do_a && o[j] >= 0 && o[j] < Be;
+        const Rows r = rows_of(ce, cx, ye[j]);
+        for (int c = 0; c < 3; ++c) hn[j][c] = in[j].h[c];
+        for (int c = 0; c < 6; ++c) pse[j][c] = in[j].ps[c];
+        iex[j] = in[j].ie;
+        const float dx1 = hn[j][1] - hp0[j][1], dx2 = hn[j][2] - hp0[j][2];
+        const float dy0 = hn[j][0] - in[j].yx, dy2 = hn[j][2] - in[j].yz;
+        const float dz0 = hn[j][0] - in[j].zx, dz1 = hn[j][1] - in[j].zy;
+        if (pa.hist && on && x >= x0 && x < x1 && o[j] >= A && o[j] < B)
+          record_curl_at(pa, t, x, y[j], z[j], dx1, dx2, dy0, dy2, dz0, dz1);
+        float dxz, dxy, dyx, dyz, dzy, dzx;
+        partials(r, dx1, dx2, dy0, dy2, dz0, dz1, pse[j], dxz, dxy, dyx, dyz, dzy, dzx);
+        float vx = in[j].e[0] + (S * (dyz - dzy)) * iex[j];
+        float vy = in[j].e[1] + (S * (dzx - dxz)) * iex[j];
+        float vz = in[j].e[2] + (S * (dxy - dyx)) * iex[j];
+        if (src1 && x == a.source_x) {
+          vy += ((sh_y[j] * wh1y) * S) * iex[j];
+          vz -= ((sh_z[j] * wh1z) * S) * iex[j];
         }
+        en[j][0] = on ? vx : 0.f;
+        en[j][1] = on ? vy : 0.f;
+        en[j][2] = on ? vz : 0.f;
         const int s = threadIdx.x + T * j;
         le1[x & 1][0][s] = en[j][0];
         le1[x & 1][1][s] = en[j][1];
         le1[x & 1][2][s] = en[j][2];
       }
       // (b) H1 of plane x - 1 over [A - nz, B + nz)
-      const int xh = x - 1;
 #pragma unroll
       for (int j = 0; j < J; ++j) {
-        for (int c = 0; c < 3; ++c) h1n[j][c] = 0.f;
-        for (int c = 0; c < 6; ++c) psh[j][c] = 0.f;
-        if (xh >= x0 - 1 && xh >= 0 && xh < nx && o[j] >= 0 && o[j] < Bh) {
-          const int i = xh * P + o[j];
-          const int s = threadIdx.x + T * j;
-          const Rows r = rows_at(ch, rows, xh, y[j], z[j]);
-          load_psi6(a.psi_h, n, i, r, psh[j]);
-          const bool up_y = y[j] + 1 < ny, up_z = z[j] + 1 < nz;
-          const float(*prev)[SLOTS] = le1[xh & 1];
-          const float qx = up_y ? prev[0][s + nz] : 0.f;
-          const float qz = up_y ? prev[2][s + nz] : 0.f;
-          const float rx = up_z ? prev[0][s + 1] : 0.f;
-          const float ry = up_z ? prev[1][s + 1] : 0.f;
-          float dxz, dxy, dyx, dyz, dzy, dzx;
-          partials(r, en[j][1] - e1p[j][1], en[j][2] - e1p[j][2], qx - e1p[j][0],
-                   qz - e1p[j][2], rx - e1p[j][0], ry - e1p[j][1], psh[j], dxz, dxy, dyx, dyz,
-                   dzy, dzx);
-          float vx = hp0[j][0] - S * (dyz - dzy);
-          float vy = hp0[j][1] - S * (dzx - dxz);
-          float vz = hp0[j][2] - S * (dxy - dyx);
-          if (src1 && xh == a.source_x) {
-            vy -= (a.src_e[2 * P + o[j]] * we1y) * S;
-            vz += (a.src_e[P + o[j]] * we1z) * S;
-          }
-          h1n[j][0] = vx;
-          h1n[j][1] = vy;
-          h1n[j][2] = vz;
-          if (a.nstage && o[j] >= A && o[j] < B && xh >= x0 && xh < x1)
-            stage_cell(a.st, a.nstage, t % a.slots, xh, y[j], z[j], e1p[j][0], e1p[j][1],
-                       e1p[j][2], vx, vy, vz, hp0[j][0], hp0[j][1], hp0[j][2]);
-        }
+        const bool on = do_b && o[j] >= 0 && o[j] < Bh;
         const int s = threadIdx.x + T * j;
+        const Rows r = rows_of(ch, cxh, yh[j]);
+        // (the clamps only touch slots past the range, whose results are zeroed)
+        const int sy = min(s + nz, SLOTS - 1), sz = min(s + 1, SLOTS - 1);
+        const bool up_y = y[j] + 1 < ny, up_z = z[j] + 1 < nz;
+        const float(*prev)[SLOTS] = le1[xh & 1];
+        const float qx = up_y ? prev[0][sy] : 0.f;
+        const float qz = up_y ? prev[2][sy] : 0.f;
+        const float rx = up_z ? prev[0][sz] : 0.f;
+        const float ry = up_z ? prev[1][sz] : 0.f;
+        float dxz, dxy, dyx, dyz, dzy, dzx;
+        partials(r, en[j][1] - e1p[j][1], en[j][2] - e1p[j][2], qx - e1p[j][0],
+                 qz - e1p[j][2], rx - e1p[j][0], ry - e1p[j][1], psh[j], dxz, dxy, dyx, dyz,
+                 dzy, dzx);
+        float vx = hp0[j][0] - S * (dyz - dzy);
+        float vy = hp0[j][1] - S * (dzx - dxz);
+        float vz = hp0[j][2] - S * (dxy - dyx);
+        if (src1 && xh == a.source_x) {
+          vy -= (se_y[j] * we1y) * S;
+          vz += (se_z[j] * we1z) * S;
+        }
+        h1n[j][0] = on ? vx : 0.f;
+        h1n[j][1] = on ? vy : 0.f;
+        h1n[j][2] = on ? vz : 0.f;
+        if (a.nstage && on && o[j] >= A && o[j] < B && xh >= x0 && xh < x1)
+          stage_cell(a.st, a.nstage, t % a.slots, xh, y[j], z[j], e1p[j][0], e1p[j][1],
+                     e1p[j][2], vx, vy, vz, hp0[j][0], hp0[j][1], hp0[j][2]);
         lh1[0][s] = h1n[j][0];
         lh1[1][s] = h1n[j][1];
         lh1[2][s] = h1n[j][2];
@@ -1087,83 +1149,80 @@ __global__ __launch_bounds__(T) void k_yee_pass2(PassArgs pa) {
       // (c) E2 of plane x - 1 over [A, B + nz)
 #pragma unroll
       for (int j = 0; j < J; ++j) {
-        for (int c = 0; c < 3; ++c) e2n[j][c] = 0.f;
-        if (xh >= x0 && xh <= x1 && xh < nx && o[j] >= A && o[j] < Bh) {
-          const int i = xh * P + o[j];
-          const int s = threadIdx.x + T * j;
-          const Rows r = rows_at(ce, rows, xh, y[j], z[j]);
-          const float yx = y[j] > 0 ? lh1[0][s - nz] : 0.f;
-          const float yz = y[j] > 0 ? lh1[2][s - nz] : 0.f;
-          const float zx = z[j] > 0 ? lh1[0][s - 1] : 0.f;
-          const float zy = z[j] > 0 ? lh1[1][s - 1] : 0.f;
-          const float dx1 = h1n[j][1] - h1p[j][1], dx2 = h1n[j][2] - h1p[j][2];
-          const float dy0 = h1n[j][0] - yx, dy2 = h1n[j][2] - yz;
-          const float dz0 = h1n[j][0] - zx, dz1 = h1n[j][1] - zy;
-          if (pa.hist && xh < x1 && o[j] < B)
-            record_curl_at(pa, t + 1, xh, y[j], z[j], dx1, dx2, dy0, dy2, dz0, dz1);
-          float dxz, dxy, dyx, dyz, dzy, dzx;
-          partials(r, dx1, dx2, dy0, dy2, dz0, dz1, pse1[j], dxz, dxy, dyx, dyz, dzy, dzx);
-          float vx = e1p[j][0] + (S * (dyz - dzy)) * ie1[j];
-          float vy = e1p[j][1] + (S * (dzx - dxz)) * ie1[j];
-          float vz = e1p[j][2] + (S * (dxy - dyx)) * ie1[j];
-          if (src2 && xh == a.source_x) {
-            vy += ((a.src_h[2 * P + o[j]] * wh2y) * S) * ie1[j];
-            vz -= ((a.src_h[P + o[j]] * wh2z) * S) * ie1[j];
-          }
-          e2n[j][0] = vx;
-          e2n[j][1] = vy;
-          e2n[j][2] = vz;
-          if (o[j] < B && xh < x1) {
-            pa.e_out[i] = vx;
-            pa.e_out[n + i] = vy;
-            pa.e_out[2 * n + i] = vz;
-            store_psi6(pa.pe_out, n, i, r, pse1[j]);
-          }
-        }
+        const bool on = do_c && o[j] >= A && o[j] < Bh;
+        const int v = on && o[j] < B && xh < x1 ? (xh * P + o[j]) * 4 : kAway;  // stored
         const int s = threadIdx.x + T * j;
+        const Rows r = rows_of(ce, cxh, ye[j]);
+        // (slots below A read clamped neighbours; their results are zeroed)
+        const int sy = max(s - nz, 0), sz = max(s - 1, 0);
+        const float yx = y[j] > 0 ? lh1[0][sy] : 0.f;
+        const float yz = y[j] > 0 ? lh1[2][sy] : 0.f;
+        const float zx = z[j] > 0 ? lh1[0][sz] : 0.f;
+        const float zy = z[j] > 0 ? lh1[1][sz] : 0.f;
+        const float dx1 = h1n[j][1] - h1p[j][1], dx2 = h1n[j][2] - h1p[j][2];
+        const float dy0 = h1n[j][0] - yx, dy2 = h1n[j][2] - yz;
+        const float dz0 = h1n[j][0] - zx, dz1 = h1n[j][1] - zy;
+        if (pa.hist && on && xh < x1 && o[j] < B)
+          record_curl_at(pa, t + 1, xh, y[j], z[j], dx1, dx2, dy0, dy2, dz0, dz1);
+        float dxz, dxy, dyx, dyz, dzy, dzx;
+        partials(r, dx1, dx2, dy0, dy2, dz0, dz1, pse1[j], dxz, dxy, dyx, dyz, dzy, dzx);
+        float vx = e1p[j][0] + (S * (dyz - dzy)) * ie1[j];
+        float vy = e1p[j][1] + (S * (dzx - dxz)) * ie1[j];
+        float vz = e1p[j][2] + (S * (dxy - dyx)) * ie1[j];
+        if (src2 && xh == a.source_x) {
+          vy += ((sh_y[j] * wh2y) * S) * ie1[j];
+          vz -= ((sh_z[j] * wh2z) * S) * ie1[j];
+        }
+        e2n[j][0] = on ? vx : 0.f;
+        e2n[j][1] = on ? vy : 0.f;
+        e2n[j][2] = on ? vz : 0.f;
+        f.e1.set(0, v, vx);
+        f.e1.set(1, v, vy);
+        f.e1.set(2, v, vz);
+        store_psi6(f.pe1, v, r, pse1[j]);
         le2[xh & 1][0][s] = e2n[j][0];
         le2[xh & 1][1][s] = e2n[j][1];
         le2[xh & 1][2][s] = e2n[j][2];
       }
       // (d) H2 of plane x - 2 over the tile
-      const int xq = x - 2;
 #pragma unroll
       for (int j = 0; j < J; ++j) {
-        if (xq >= x0 && xq < x1 && o[j] >= A && o[j] < B) {
-          const int i = xq * P + o[j];
-          const int s = threadIdx.x + T * j;
-          const Rows r = rows_at(ch, rows, xq, y[j], z[j]);
-          const bool up_y = y[j] + 1 < ny, up_z = z[j] + 1 < nz;
-          const float(*prev)[SLOTS] = le2[xq & 1];
-          const float qx = up_y ? prev[0][s + nz] : 0.f;
-          const float qz = up_y ? prev[2][s + nz] : 0.f;
-          const float rx = up_z ? prev[0][s + 1] : 0.f;
-          const float ry = up_z ? prev[1][s + 1] : 0.f;
-          float dxz, dxy, dyx, dyz, dzy, dzx;
-          partials(r, e2n[j][1] - e2p[j][1], e2n[j][2] - e2p[j][2], qx - e2p[j][0],
-                   qz - e2p[j][2], rx - e2p[j][0], ry - e2p[j][1], psh1[j], dxz, dxy, dyx, dyz,
-                   dzy, dzx);
-          float vx = h1p[j][0] - S * (dyz - dzy);
-          float vy = h1p[j][1] - S * (dzx - dxz);
-          float vz = h1p[j][2] - S * (dxy - dyx);
-          if (src2 && xq == a.source_x) {
-            vy -= (a.src_e[2 * P + o[j]] * we2y) * S;
-            vz += (a.src_e[P + o[j]] * we2z) * S;
-          }
-          if (save && xq >= a.hbox[0] && xq < a.hbox[1] && y[j] >= a.hbox[2] &&
-              y[j] < a.hbox[3] && z[j] >= a.hbox[4] && z[j] < a.hbox[5]) {
-            a.hprev[i] = h1p[j][0];
-            a.hprev[n + i] = h1p[j][1];
-            a.hprev[2 * n + i] = h1p[j][2];
-          }
-          pa.h_out[i] = vx;
-          pa.h_out[n + i] = vy;
-          pa.h_out[2 * n + i] = vz;
-          store_psi6(pa.ph_out, n, i, r, psh1[j]);
-          if (a.nstage)
-            stage_cell(a.st, a.nstage, (t + 1) % a.slots, xq, y[j], z[j], e2p[j][0], e2p[j][1],
-                       e2p[j][2], vx, vy, vz, h1p[j][0], h1p[j][1], h1p[j][2]);
+        const bool on = do_d && o[j] >= A && o[j] < B;
+        const int v = on ? (xq * P + o[j]) * 4 : kAway;
+        const int s = threadIdx.x + T * j;
+        const Rows r = rows_of(ch, cxq, yh[j]);
+        const int sy = min(s + nz, SLOTS - 1), sz = min(s + 1, SLOTS - 1);
+        const bool up_y = y[j] + 1 < ny, up_z = z[j] + 1 < nz;
+        const float(*prev)[SLOTS] = le2[xq & 1];
+        const float qx = up_y ? prev[0][sy] : 0.f;
+        const float qz = up_y ? prev[2][sy] : 0.f;
+        const float rx = up_z ? prev[0][sz] : 0.f;
+        const float ry = up_z ? prev[1][sz] : 0.f;
+        float dxz, dxy, dyx, dyz, dzy, dzx;
+        partials(r, e2n[j][1] - e2p[j][1], e2n[j][2] - e2p[j][2], qx - e2p[j][0],
+                 qz - e2p[j][2], rx - e2p[j][0], ry - e2p[j][1], psh1[j], dxz, dxy, dyx, dyz,
+                 dzy, dzx);
+        float vx = h1p[j][0] - S * (dyz - dzy);
+        float vy = h1p[j][1] - S * (dzx - dxz);
+        float vz = h1p[j][2] - S * (dxy - dyx);
+        if (src2 && xq == a.source_x) {
+          vy -= (se_y[j] * we2y) * S;
+          vz += (se_z[j] * we2z) * S;
         }
+        if (on && save && xq >= a.hbox[0] && xq < a.hbox[1] && y[j] >= a.hbox[2] &&
+            y[j] < a.hbox[3] && z[j] >= a.hbox[4] && z[j] < a.hbox[5]) {
+          const int i = xq * P + o[j];
+          a.hprev[i] = h1p[j][0];
+          a.hprev[a.n + i] = h1p[j][1];
+          a.hprev[2 * a.n + i] = h1p[j][2];
+        }
+        f.h1.set(0, v, vx);
+        f.h1.set(1, v, vy);
+        f.h1.set(2, v, vz);
+        store_psi6(f.ph1, v, r, psh1[j]);
+        if (a.nstage && on)
+          stage_cell(a.st, a.nstage, (t + 1) % a.slots, xq, y[j], z[j], e2p[j][0], e2p[j][1],
+                     e2p[j][2], vx, vy, vz, h1p[j][0], h1p[j][1], h1p[j][2]);
       }
 #pragma unroll
       for (int j = 0; j < J; ++j) {
@@ -1291,6 +1350,31 @@ int launch_update(float* f, const float* g, const float* inv_eps, float* psi,
   return WFD_OK;
 }
 
+// x planes per workgroup for a pass shape: a workgroup marches its chunk
+// with a barrier per plane, so the pass takes about (rounds of co-resident
+// workgroups per XCD) x (planes a march visits: the chunk plus one halo plane
+// per step).  The count minimising that, from the kernel's occupancy.
+int pass_planes(const void* kernel, int threads, int tiles, int nx, int steps) {
+  int dev = 0, cus = 0, occ = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, threads, 0) != hipSuccess)
+    return 8;
+  const int64_t per_xcd = (int64_t)max(cus, 8) / 8 * max(occ, 1);
+  int best = 1;
+  int64_t best_cost = INT64_MAX;
+  for (int xc = 1; xc <= min(64, nx); ++xc) {
+    const int64_t wgs = (int64_t)tiles * ((nx + xc - 1) / xc);
+    const int64_t rounds = ((wgs + 7) / 8 + per_xcd - 1) / per_xcd;
+    const int64_t cost = rounds * (xc + steps);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = xc;
+    }
+  }
+  return best;
+}
+
 }  // namespace
 }  // namespace wfd
 
@@ -1354,18 +1438,27 @@ int wfd_yee_pass(const wfd_yee_plan* p, const wfd_yee_targets* out, void* stream
   if (int status = build_args(p, pa.a, "yee_pass")) return status;
   WFD_REQUIRE(out && out->e && out->h && out->psi_e && out->psi_h && out->ticket,
               "yee_pass: null target or ticket pointer");
+  // psi (6 components) is addressed through 32-bit buffer offsets
+  WFD_REQUIRE((int64_t)24 * pa.a.n <= 0x7ff00000,
+              "yee_pass: grids above %d cells step with wfd_yee_step", 0x7ff00000 / 24);
   WFD_REQUIRE(out->e != p->e && out->h != p->h && out->psi_e != p->psi_e &&
                   out->psi_h != p->psi_h,
               "yee_pass: targets must be buffers other than the plan's");
   const int steps = out->steps ? out->steps : 1;
   WFD_REQUIRE(steps == 1 || steps == 2, "yee_pass: steps must be 1 or 2");
-  // one step: 512 threads x 1 offset or 256 threads x 2 or 4; two steps:
-  // 512 threads x 1 or 2
+  // one step: 512 or 1024 threads x 1 offset, 256 threads x 2 or 4; two
+  // steps: 512 threads x 1 or 2, 1024 threads x 1
   const int per_thread = out->slots_per_thread ? out->slots_per_thread : (steps == 1 ? 4 : 2);
   WFD_REQUIRE(steps == 1 ? (per_thread == 1 || per_thread == 2 || per_thread == 4)
                          : (per_thread == 1 || per_thread == 2),
               "yee_pass: slots_per_thread must be 1, 2 or 4 (one step), 1 or 2 (two steps)");
-  const int threads = (steps == 2 || per_thread == 1) ? 2 * kBlock : kBlock;
+  const int threads = out->threads ? out->threads
+                                   : (steps == 2 || per_thread == 1) ? 2 * kBlock : kBlock;
+  WFD_REQUIRE(threads == 4 * kBlock ? per_thread == 1
+                                    : threads == ((steps == 2 || per_thread == 1) ? 2 * kBlock
+                                                                                  : kBlock),
+              "yee_pass: %d threads with %d slots per thread is not a pass shape", threads,
+              per_thread);
   const int slots = per_thread * threads;
   const int halo = steps == 1 ? p->nz : 3 * p->nz;
   WFD_REQUIRE(halo < slots, "yee_pass: nz = %d leaves no tile in %d slots per workgroup", p->nz,
@@ -1391,9 +1484,15 @@ int wfd_yee_pass(const wfd_yee_plan* p, const wfd_yee_targets* out, void* stream
   pa.tiles = (pa.a.P + pa.W - 1) / pa.W;
   int xc = out->planes;
   if (xc == 0) {
-    // about 2048 workgroups, at least 4 planes per chunk
-    xc = (int)(((int64_t)pa.tiles * p->nx + 2047) / 2048);
-    xc = max(4, min(16, xc));
+    const void* kernel =
+        steps == 2 ? (threads == 4 * kBlock ? (const void*)&k_yee_pass2<4 * kBlock, 1>
+                      : per_thread == 1     ? (const void*)&k_yee_pass2<2 * kBlock, 1>
+                                            : (const void*)&k_yee_pass2<2 * kBlock, 2>)
+        : threads == 4 * kBlock ? (const void*)&k_yee_pass<4 * kBlock, 1>
+        : per_thread == 1       ? (const void*)&k_yee_pass<2 * kBlock, 1>
+        : per_thread == 2       ? (const void*)&k_yee_pass<kBlock, 2>
+                                : (const void*)&k_yee_pass<kBlock, 4>;
+    xc = pass_planes(kernel, threads, pa.tiles, p->nx, steps);
   }
   pa.xc = xc;
   const int chunks = (p->nx + xc - 1) / xc;
@@ -1402,10 +1501,14 @@ int wfd_yee_pass(const wfd_yee_plan* p, const wfd_yee_targets* out, void* stream
   hipStream_t s = as_stream(stream);
   const dim3 grid(pa.per * 8);
   if (steps == 2) {
-    if (per_thread == 1)
+    if (threads == 4 * kBlock)
+      hipLaunchKernelGGL((k_yee_pass2<4 * kBlock, 1>), grid, dim3(threads), 0, s, pa);
+    else if (per_thread == 1)
       hipLaunchKernelGGL((k_yee_pass2<2 * kBlock, 1>), grid, dim3(threads), 0, s, pa);
     else
       hipLaunchKernelGGL((k_yee_pass2<2 * kBlock, 2>), grid, dim3(threads), 0, s, pa);
+  } else if (threads == 4 * kBlock) {
+    hipLaunchKernelGGL((k_yee_pass<4 * kBlock, 1>), grid, dim3(threads), 0, s, pa);
   } else if (per_thread == 1) {
     hipLaunchKernelGGL((k_yee_pass<2 * kBlock, 1>), grid, dim3(threads), 0, s, pa);
   } else if (per_thread == 2) {

@@ -264,7 +264,7 @@ class Solver:
         self.single_pass_mode = single_pass
         self.single_pass = False
         # (x planes per workgroup, slots per thread, steps per launch); 0 = chosen
-        self.pass_shape = (0, 0, 1)
+        self.pass_shape = (0, 0, 1, 0)
         self._alt = None          # the second buffer set of the single pass
         self._ticket = None
         self._phase = 0           # 1 while the state is in the second set (inside a capture)
@@ -406,9 +406,10 @@ class Solver:
         return (self.cpml_storage == "full" and len(self.staged()) == len(self.monitors)
                 and self.shape[2] < 1024)
 
-    def use_single_pass(self, planes=0, slots_per_thread=0, steps=1):
+    def use_single_pass(self, planes=0, slots_per_thread=0, steps=1, threads=0):
         """Step captured chunks with wfd_yee_pass, ``steps`` (1 or 2) steps per
-        launch (allocates the second buffer set)."""
+        launch, ``threads`` per workgroup (0: the library's choice for the
+        shape; allocates the second buffer set)."""
         if not self.single_pass_applies():
             raise ValueError("the single-pass step needs full CPML storage, staged monitors "
                              "only and nz < 1024")
@@ -420,7 +421,7 @@ class Solver:
                          "psi_e": Array.zeros(self.psi_e.shape, device=self.device),
                          "psi_h": Array.zeros(self.psi_h.shape, device=self.device)}
             self._ticket = Array.zeros(1, np.int32, self.device)
-        self.pass_shape = (int(planes), int(slots_per_thread), int(steps))
+        self.pass_shape = (int(planes), int(slots_per_thread), int(steps), int(threads))
         self.single_pass = True
 
     def _pass_schedule(self, length, two_step=True):
@@ -468,6 +469,7 @@ class Solver:
         out.steps = 2 if kind == "p2" else 1
         if out.steps == self.pass_shape[2]:
             out.planes, out.slots_per_thread = self.pass_shape[:2]
+            out.threads = self.pass_shape[3]
         if kind == "p2" and self._curl is not None:
             hist, slots, box = self._curl
             out.hist, out.hist_steps = ptr(hist), int(slots)
