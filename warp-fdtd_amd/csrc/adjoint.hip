@@ -211,6 +211,161 @@ __global__ __launch_bounds__(kBlock) void k_reverse_half(
   }
 }
 
+// The fused reverse half-step again, with a thread marching XC x planes of
+// one (y, z) offset (the block: 256 consecutive offsets, blocks striped over
+// the XCDs as in the forward step).  A cell's x-axis pair of prepare slots is
+// the neighbouring plane's own pair, so the march computes it once per plane
+// (plus one halo plane) instead of twice, and reads q, eps and psi of the x
+// neighbour from registers; every access is a buffer access, masked by an
+// out-of-range offset rather than a branch, and the CPML terms are selected.
+// The arithmetic of each cell is k_reverse_half's, so results are the same
+// bits.
+struct RevArgs {
+  float* q_dst;
+  const float* q_src;
+  const float* eps;
+  const float* psi_in;
+  float* psi_out;
+  const float* coeff;
+  void* grad;
+  const float* hist;
+  const int* step;
+  int rows, hist_steps;
+  int nx, ny, nz, n, P;
+  float inv_nz;
+  int bx0, by0, bz0, bsx, bsy, bsz;  // design box (history and gradient)
+  float courant;
+  int segs, total, per;  // (y, z) segments per chunk, logical blocks, per XCD
+};
+
+// One axis pair of prepare slots (prepare_pair's operations) from the loaded
+// values: qa, qb the pair's q components, e the cell's eps (electric), sa, sb
+// its psi slots, c0..c2 the axis row; p1, p2 the updated psi.
+template <bool ELECTRIC>
+__device__ __forceinline__ void rev_pair(float qa, float qb, float e, float sa, float sb,
+                                         float c0, float c1, float c2, float courant,
+                                         float& d1, float& d2, float& p1, float& p2) {
+  const float factor = ELECTRIC ? courant * e : -courant;
+  const float q1 = -(factor * qa);
+  const float q2 = factor * qb;
+  const float s1 = sa + q1, s2 = sb + q2;
+  d1 = c0 != 0.f ? c2 * q1 + c0 * s1 : c2 * q1;
+  d2 = c0 != 0.f ? c2 * q2 + c0 * s2 : c2 * q2;
+  p1 = c1 * s1;
+  p2 = c1 * s2;
+}
+
+template <typename G, int XC, bool FROM_H>
+__global__ __launch_bounds__(kBlock) void k_reverse_march(RevArgs r) {
+  constexpr bool E = !FROM_H;           // prepare is electric when q_src is qe
+  constexpr int D = FROM_H ? -1 : 1;    // the gather's neighbour along each axis
+  const int logical = (blockIdx.x & 7) * r.per + (blockIdx.x >> 3);
+  if (logical >= r.total) return;
+  const int chunk = logical / r.segs;
+  const int o = (logical - chunk * r.segs) * kBlock + threadIdx.x;
+  if (o >= r.P) return;
+  const int nx = r.nx, ny = r.ny, nz = r.nz, n = r.n, P = r.P, rows = r.rows;
+  const int x0 = chunk * XC;
+  const int y = div_small(o, nz, r.inv_nz), z = o - y * nz;
+  const Field qs(r.q_src, 3, n), qd(r.q_dst, 3, n), pin(r.psi_in, 6, n), pout(r.psi_out, 6, n);
+  const Field ep(E ? r.eps : r.q_src, 1, n);  // (unread when FROM_H)
+  const float* __restrict__ co = r.coeff;
+  const float S = r.courant;
+  // y and z rows of the cell and of its neighbours
+  const int yn = y + D, zn = z + D;
+  const bool yok = yn >= 0 && yn < ny, zok = zn >= 0 && zn < nz;
+  const int ync = yok ? yn : y, znc = zok ? zn : z;
+  const float cy0 = co[(rows + y) * 3], cy1 = co[(rows + y) * 3 + 1],
+              cy2 = co[(rows + y) * 3 + 2];
+  const float cz0 = co[(2 * rows + z) * 3], cz1 = co[(2 * rows + z) * 3 + 1],
+              cz2 = co[(2 * rows + z) * 3 + 2];
+  const float ny0 = co[(rows + ync) * 3], ny1 = co[(rows + ync) * 3 + 1],
+              ny2 = co[(rows + ync) * 3 + 2];
+  const float nz0 = co[(2 * rows + znc) * 3], nz1 = co[(2 * rows + znc) * 3 + 1],
+              nz2 = co[(2 * rows + znc) * 3 + 2];
+  // the x-axis pair of planes x0 - 1 + k (D < 0) or x0 + k (D > 0), k = 0..XC
+  float xd1[XC + 1], xd2[XC + 1];
+  constexpr int kbase = D < 0 ? -1 : 0;
+#pragma unroll
+  for (int k = 0; k <= XC; ++k) {
+    const int x = x0 + kbase + k;
+    const bool on = x >= 0 && x < nx;  // uniform
+    const int xc = on ? x : 0;
+    const int v = on ? (xc * P + o) * 4 : kAway;
+    const float c0 = co[xc * 3], c1 = co[xc * 3 + 1], c2 = co[xc * 3 + 2];
+    const int vp = c0 != 0.f ? v : kAway;
+    float p1, p2;
+    rev_pair<E>(qs.at(1, v), qs.at(2, v), E ? ep.at(0, v) : 0.f, pin.at(0, vp), pin.at(1, vp),
+                c0, c1, c2, S, xd1[k], xd2[k], p1, p2);
+    // planes of the chunk write their updated x psi; planes past the grid
+    // contribute zero
+    if (!on) xd1[k] = xd2[k] = 0.f;
+    const bool mine = on && x >= x0 && x < x0 + XC;
+    pout.set(0, mine ? vp : kAway, p1);
+    pout.set(1, mine ? vp : kAway, p2);
+  }
+  const int t = r.step[0];
+#pragma unroll
+  for (int p = 0; p < XC; ++p) {
+    const int x = x0 + p;
+    if (x >= nx) break;  // uniform
+    const int i = x * P + o;
+    const int v = i * 4;
+    const int ko = p - kbase, kn = ko + D;  // own and neighbour x pairs
+    const float own0 = xd1[ko], own1 = xd2[ko], nb0 = xd1[kn], nb1 = xd2[kn];
+    // y axis: components 2, 0; slots 2, 3
+    float own2, own3, nb2 = 0.f, nb3 = 0.f, p1, p2, u1, u2;
+    const float ev = E ? ep.at(0, v) : 0.f;
+    const float q0 = qs.at(0, v), q1 = qs.at(1, v), q2 = qs.at(2, v);
+    const int vy = cy0 != 0.f ? v : kAway;
+    rev_pair<E>(q2, q0, ev, pin.at(2, vy), pin.at(3, vy), cy0, cy1, cy2, S, own2, own3, p1, p2);
+    pout.set(2, vy, p1);
+    pout.set(3, vy, p2);
+    {
+      const int w = yok ? v + D * nz * 4 : kAway;
+      const int wp = ny0 != 0.f ? w : kAway;
+      rev_pair<E>(qs.at(2, w), qs.at(0, w), E ? ep.at(0, w) : 0.f, pin.at(2, wp), pin.at(3, wp),
+                  ny0, ny1, ny2, S, nb2, nb3, u1, u2);
+      if (!yok) nb2 = nb3 = 0.f;
+    }
+    // z axis: components 0, 1; slots 4, 5
+    float own4, own5, nb4 = 0.f, nb5 = 0.f;
+    const int vz = cz0 != 0.f ? v : kAway;
+    rev_pair<E>(q0, q1, ev, pin.at(4, vz), pin.at(5, vz), cz0, cz1, cz2, S, own4, own5, p1, p2);
+    pout.set(4, vz, p1);
+    pout.set(5, vz, p2);
+    {
+      const int w = zok ? v + D * 4 : kAway;
+      const int wp = nz0 != 0.f ? w : kAway;
+      rev_pair<E>(qs.at(0, w), qs.at(1, w), E ? ep.at(0, w) : 0.f, pin.at(4, wp), pin.at(5, wp),
+                  nz0, nz1, nz2, S, nb4, nb5, u1, u2);
+      if (!zok) nb4 = nb5 = 0.f;
+    }
+    // the gather (k_reverse_half's sums, in its order)
+    float v0 = (nb2 - own2) + (nb5 - own5);
+    float v1 = (nb1 - own1) + (nb4 - own4);
+    float v2 = (nb0 - own0) + (nb3 - own3);
+    if (!FROM_H) {
+      v0 = -v0;
+      v1 = -v1;
+      v2 = -v2;
+    }
+    const float a0 = qd.at(0, v) + v0, a1 = qd.at(1, v) + v1, a2 = qd.at(2, v) + v2;
+    qd.set(0, v, a0);
+    qd.set(1, v, a1);
+    qd.set(2, v, a2);
+    if (FROM_H && t >= 0 && x >= r.bx0 && x < r.bx0 + r.bsx && y >= r.by0 &&
+        y < r.by0 + r.bsy && z >= r.bz0 && z < r.bz0 + r.bsz) {
+      const int dn = r.bsx * r.bsy * r.bsz;
+      const int j = ((x - r.bx0) * r.bsy + (y - r.by0)) * r.bsz + (z - r.bz0);
+      const int64_t base = (int64_t)(t % r.hist_steps) * 3 * dn + j;
+      const float contrib =
+          r.hist[base] * a0 + r.hist[base + dn] * a1 + r.hist[base + 2 * dn] * a2;
+      static_cast<G*>(r.grad)[j] += (G)contrib;
+    }
+  }
+}
+
 // adjoint.py:165-223 — transposed colocation.  The reference scatters each
 // plane cell's six weighted values with atomics; here every target cell of
 // the stencil footprint gathers the same contributions in a fixed order, so
@@ -429,10 +584,49 @@ int launch_reverse_half(float* q_dst, const float* q_src, const float* eps, cons
   WFD_REQUIRE(!(from_h && has_box) || (hist && grad && hist_steps > 0),
               "reverse_half: history and gradient required");
   const int n = nx * ny * nz;
-  hipLaunchKernelGGL((k_reverse_half<G>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                     as_stream(stream), q_dst, q_src, eps, psi_in, psi_out, coeff, coeff_rows,
-                     hist, hist_steps > 0 ? hist_steps : 1, grad, step, nx, ny, nz, x0, y0, z0,
-                     sx, sy, sz, courant, from_h);
+  hipStream_t s = as_stream(stream);
+  if ((int64_t)24 * n > kFieldBytes) {  // past 32-bit buffer offsets: one thread per cell
+    hipLaunchKernelGGL((k_reverse_half<G>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                       q_dst, q_src, eps, psi_in, psi_out, coeff, coeff_rows, hist,
+                       hist_steps > 0 ? hist_steps : 1, grad, step, nx, ny, nz, x0, y0, z0, sx,
+                       sy, sz, courant, from_h);
+    WFD_LAUNCH_CHECK("reverse_half");
+    return WFD_OK;
+  }
+  constexpr int XC = 4;
+  RevArgs r;
+  r.q_dst = q_dst;
+  r.q_src = q_src;
+  r.eps = eps;
+  r.psi_in = psi_in;
+  r.psi_out = psi_out;
+  r.coeff = coeff;
+  r.grad = grad;
+  r.hist = hist;
+  r.step = step;
+  r.rows = coeff_rows;
+  r.hist_steps = hist_steps > 0 ? hist_steps : 1;
+  r.nx = nx;
+  r.ny = ny;
+  r.nz = nz;
+  r.n = n;
+  r.P = ny * nz;
+  r.inv_nz = 1.0f / (float)nz;
+  r.bx0 = x0;
+  r.by0 = y0;
+  r.bz0 = z0;
+  r.bsx = sx;
+  r.bsy = sy;
+  r.bsz = sz;
+  r.courant = courant;
+  r.segs = (r.P + kBlock - 1) / kBlock;
+  r.total = r.segs * ((nx + XC - 1) / XC);
+  r.per = (r.total + 7) / 8;
+  const dim3 grid(r.per * 8);
+  if (from_h)
+    hipLaunchKernelGGL((k_reverse_march<G, XC, true>), grid, dim3(kBlock), 0, s, r);
+  else
+    hipLaunchKernelGGL((k_reverse_march<G, XC, false>), grid, dim3(kBlock), 0, s, r);
   WFD_LAUNCH_CHECK("reverse_half");
   return WFD_OK;
 }

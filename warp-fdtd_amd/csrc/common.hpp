@@ -68,6 +68,45 @@ __device__ __forceinline__ void st(Rsrc r, float v, int voff, int soff) {
   __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
 }
 
+// One float array of components of n cells each (E, H: 3; psi: 6; 1/eps: 1)
+// through a buffer resource: every access of a cell shares its 32-bit byte
+// offset, the component adds a uniform offset, so a lane holds one offset
+// register per cell instead of a 64-bit address per load.  The array must
+// span fewer than 2^31 bytes (checked on the host).
+struct Field {
+  Rsrc r;
+  int cb;  // bytes per component
+  __device__ __forceinline__ Field(const float* p, int comps, int n)
+      : r(buffer(p, (unsigned)comps * (unsigned)n * 4u)), cb(n * 4) {}
+  __device__ __forceinline__ float operator()(int c, int i) const { return ld(r, i * 4, c * cb); }
+  __device__ __forceinline__ void put(int c, int i, float v) const { st(r, v, i * 4, c * cb); }
+  // by byte offset: kAway reads 0 and drops the store, so a masked access
+  // needs no branch
+  __device__ __forceinline__ float at(int c, int voff) const { return ld(r, voff, c * cb); }
+  __device__ __forceinline__ void set(int c, int voff, float v) const { st(r, v, voff, c * cb); }
+};
+
+// A byte offset past the end of every Field (hosts keep the arrays a kernel
+// addresses this way below 0x7ff00000 bytes, so even with a component offset
+// added it stays out of range without wrapping).
+constexpr int64_t kFieldBytes = 0x7ff00000;
+constexpr int kAway = 0x7ffffff0;
+
+// A buffer's base address as a value the compiler cannot see through: a
+// Field built from it inside a loop is rebuilt there (a few scalar moves)
+// instead of being hoisted, so that many live buffer descriptors do not
+// spill the scalar registers.
+__device__ __forceinline__ const float* opaque(const float* p) {
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
+// a where the mask is all ones, else b (bitwise, no lane-mask register)
+__device__ __forceinline__ float pick(int m, float a, float b) {
+  return __int_as_float((__float_as_int(a) & m) | (__float_as_int(b) & ~m));
+}
+__device__ __forceinline__ int pick(int m, int a, int b) { return (a & m) | (b & ~m); }
+
 // Device copy of a wfd_stage footprint (box plus its cell count).
 struct StageBox {
   float* buf;

@@ -484,49 +484,11 @@ __global__ __launch_bounds__(kBlock) void k_batch_h(const YeeArgs* __restrict__ 
   yee_h<XC>(table[d], blockIdx.x - first[d]);
 }
 
-// One float array of components of n cells each (E, H: 3; psi: 6; 1/eps: 1)
-// through a buffer resource: every access of a cell shares its 32-bit byte
-// offset, the component adds a uniform offset, so a lane holds one offset
-// register per cell instead of a 64-bit address per load.  The array must
-// span fewer than 2^31 bytes (checked on the host).
-struct Field {
-  Rsrc r;
-  int cb;  // bytes per component
-  __device__ __forceinline__ Field(const float* p, int comps, int n)
-      : r(buffer(p, (unsigned)comps * (unsigned)n * 4u)), cb(n * 4) {}
-  __device__ __forceinline__ float operator()(int c, int i) const { return ld(r, i * 4, c * cb); }
-  __device__ __forceinline__ void put(int c, int i, float v) const { st(r, v, i * 4, c * cb); }
-  // by byte offset: kAway reads 0 and drops the store, so a masked access
-  // needs no branch
-  __device__ __forceinline__ float at(int c, int voff) const { return ld(r, voff, c * cb); }
-  __device__ __forceinline__ void set(int c, int voff, float v) const { st(r, v, voff, c * cb); }
-};
-
-// A byte offset past the end of every Field (the host keeps the arrays of a
-// pass below 0x7ff00000 bytes, so even with a component offset added it
-// stays out of range without wrapping).
-constexpr int kAway = 0x7ffffff0;
-
-// A buffer's base address as a value the compiler cannot see through: a
-// Field built from it inside a loop is rebuilt there (a few scalar moves)
-// instead of being hoisted, which keeps nine live buffer descriptors from
-// spilling the scalar registers.
-__device__ __forceinline__ const float* opaque(const float* p) {
-  asm volatile("" : "+s"(p));
-  return p;
-}
-
 // The CPML coefficient rows of one cell (solver.py's coeff tables).
 struct Rows {
   float xa, xb, xk, ya, yb, yk, za, zb, zk;
   int my, mz;  // lane masks: all ones where the y / z row has CPML
 };
-
-// a where the mask is all ones, else b (bitwise, no lane-mask register)
-__device__ __forceinline__ float pick(int m, float a, float b) {
-  return __int_as_float((__float_as_int(a) & m) | (__float_as_int(b) & ~m));
-}
-__device__ __forceinline__ int pick(int m, int a, int b) { return (a & m) | (b & ~m); }
 
 // The six CPML partials of one cell (kernels.py:29-47, the fused step's
 // operations); ps[] becomes the updated psi where the axis has CPML.  Both
@@ -1439,8 +1401,8 @@ int wfd_yee_pass(const wfd_yee_plan* p, const wfd_yee_targets* out, void* stream
   WFD_REQUIRE(out && out->e && out->h && out->psi_e && out->psi_h && out->ticket,
               "yee_pass: null target or ticket pointer");
   // psi (6 components) is addressed through 32-bit buffer offsets
-  WFD_REQUIRE((int64_t)24 * pa.a.n <= 0x7ff00000,
-              "yee_pass: grids above %d cells step with wfd_yee_step", 0x7ff00000 / 24);
+  WFD_REQUIRE((int64_t)24 * pa.a.n <= kFieldBytes,
+              "yee_pass: grids above %d cells step with wfd_yee_step", (int)(kFieldBytes / 24));
   WFD_REQUIRE(out->e != p->e && out->h != p->h && out->psi_e != p->psi_e &&
                   out->psi_h != p->psi_h,
               "yee_pass: targets must be buffers other than the plan's");
