@@ -180,6 +180,21 @@ int wfd_batch_prepare(const wfd_yee_plan* plans, int count, void* workspace,
                       int64_t workspace_bytes, void* stream);
 int wfd_batch_step(const wfd_yee_plan* plans, int count, const void* workspace, void* stream);
 
+/* Two-step passes (wfd_yee_pass, steps = 2, 1024 threads x 1 slot) of
+ * several grids in one launch: plans[d] is pass d's start and targets[d] its
+ * result (the buffer set it writes, its own ticket), as for wfd_yee_pass;
+ * planes (equal in every target; 0: chosen from the occupancy for the whole
+ * launch) is common to all.  Each grid's result is bitwise that of its own
+ * wfd_yee_pass.  prepare writes the argument table into the workspace
+ * (workspace_bytes from wfd_batch_pass_workspace_bytes, -1 for a bad count)
+ * and synchronizes the stream: call it outside graph capture; the launch
+ * must receive the same plans and targets.  No reference counterpart. */
+int64_t wfd_batch_pass_workspace_bytes(int count);
+int wfd_batch_pass_prepare(const wfd_yee_plan* plans, const wfd_yee_targets* targets, int count,
+                           void* workspace, int64_t workspace_bytes, void* stream);
+int wfd_batch_pass(const wfd_yee_plan* plans, const wfd_yee_targets* targets, int count,
+                   const void* workspace, void* stream);
+
 /* ------------------------------------------------- on-chip-resident stepping
  * The same step as wfd_yee_step (bitwise), for grids whose E, H and 1/eps fit
  * in the LDS of co-resident workgroups: sub-boxes of sx x sy x sz cells (one

@@ -31,17 +31,21 @@ def _solvers(pkg, cases, chunks):
     return out
 
 
-@pytest.mark.parametrize("chunks", [(1, 2, 4), (2, 2, 8)])
-def test_batch_equals_single_runs(pkg, gpu, cases, chunks):
+@pytest.mark.parametrize("chunks,single_pass", [((1, 2, 4), False), ((2, 2, 8), False),
+                                                ((1, 2, 4), True)])
+def test_batch_equals_single_runs(pkg, gpu, cases, chunks, single_pass):
     """Three shapes, mixed x-chunk classes, staged DFT planes: every design's
-    fields, CPML state, counters and phasors equal its own solo run bitwise."""
+    fields, CPML state, counters and phasors equal its own solo run bitwise,
+    on the batched fused step or on the batched two-step pass (300 steps:
+    chunks of 64 and 44, both multiples of 4, all on passes)."""
     solo = _solvers(pkg, cases, chunks)
     for s in solo:
         s.compile()
         s.reset()
         s.run()
-    batch = pkg.batch.Batch(_solvers(pkg, cases, chunks))
+    batch = pkg.batch.Batch(_solvers(pkg, cases, chunks), single_pass=single_pass)
     batch.compile()
+    assert batch.use_pass == single_pass
     batch.run()
     torch.cuda.synchronize()
     for a, b in zip(solo, batch.solvers):

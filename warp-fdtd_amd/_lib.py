@@ -138,6 +138,9 @@ SIGNATURES = {
     "wfd_batch_workspace_bytes": [I],
     "wfd_batch_prepare": [ctypes.POINTER(YeePlan), I, P, L, P],
     "wfd_batch_step": [ctypes.POINTER(YeePlan), I, P, P],
+    "wfd_batch_pass_workspace_bytes": [I],
+    "wfd_batch_pass_prepare": [ctypes.POINTER(YeePlan), ctypes.POINTER(YeeTargets), I, P, L, P],
+    "wfd_batch_pass": [ctypes.POINTER(YeePlan), ctypes.POINTER(YeeTargets), I, P, P],
     "wfd_resident_layout_for": [ctypes.POINTER(YeePlan), I, I, I, ctypes.POINTER(ResidentLayout)],
     "wfd_resident_run": [ctypes.POINTER(YeePlan), ctypes.POINTER(ResidentLayout), P, I, I, P],
     "wfd_resident_run_noalloc": [ctypes.POINTER(YeePlan), ctypes.POINTER(ResidentLayout), I, I, P,
@@ -197,6 +200,7 @@ def lib():
             fn.argtypes = args
             fn.restype = {"wfd_last_error": ctypes.c_char_p,
                           "wfd_batch_workspace_bytes": ctypes.c_int64,
+                          "wfd_batch_pass_workspace_bytes": ctypes.c_int64,
                           "wfd_colsum_scratch": ctypes.c_int64,
                           "wfd_mse_partials": ctypes.c_int64}.get(name, ctypes.c_int)
         _lib = handle

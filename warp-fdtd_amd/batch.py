@@ -10,6 +10,10 @@ and with one common chunk for all designs and keeps the fastest, writes the
 device argument tables and
 captures one HIP graph per chunk of ``graph_steps`` steps that steps every
 design and replays every design's staged detectors; ``run`` replays them.
+Where every design can take the single pass, ``compile`` also times the
+batched two-step pass (``wfd_batch_pass``: two steps of every design per
+launch, each design alternating between its own two buffer sets) and uses
+it when it is faster; chunks end back in each design's first set.
 """
 
 import time
@@ -17,13 +21,14 @@ import time
 import numpy as np
 import torch
 
-from ._lib import WFD_MAX_BATCH, Handles, YeePlan, call, ptr, release, stream
+from ._lib import WFD_MAX_BATCH, Handles, YeePlan, YeeTargets, call, ptr, release, stream
 from .array import Array
 
 
 class Batch:
-    def __init__(self, solvers, graph_steps=64):
+    def __init__(self, solvers, graph_steps=64, single_pass="auto"):
         self.solvers = list(solvers)
+        self.pass_mode = single_pass
         if not 1 <= len(self.solvers) <= WFD_MAX_BATCH:
             raise ValueError(f"A batch holds 1..{WFD_MAX_BATCH} solvers")
         steps = {s.steps for s in self.solvers}
@@ -41,6 +46,9 @@ class Batch:
         self.graphs, self.quiet_graphs = {}, {}
         self._handles = Handles()
         self._tables = {}
+        self._pass_tables = {}
+        self.use_pass = False
+        self.pass_us = None
         self.cells = sum(s.n for s in self.solvers)
 
     # ------------------------------------------------------------ tables
@@ -50,6 +58,66 @@ class Batch:
         ws = Array.zeros((nbytes + 3) // 4, np.int32, self.device)
         call("wfd_batch_prepare", plans, len(self.solvers), ptr(ws), nbytes, stream())
         return plans, ws
+
+    def _pass_table(self, record, direction):
+        """Plans and targets of the batched two-step pass from each design's
+        first buffer set to its second (direction 0) or back (1)."""
+        plans, targets = [], []
+        for solver in self.solvers:
+            plan = solver._plan(stage=record)
+            main = {"E": solver.E, "H": solver.H, "psi_e": solver.psi_e, "psi_h": solver.psi_h}
+            src, dst = (main, solver._alt) if direction == 0 else (solver._alt, main)
+            plan.e, plan.h = ptr(src["E"]), ptr(src["H"])
+            plan.psi_e, plan.psi_h = ptr(src["psi_e"]), ptr(src["psi_h"])
+            out = YeeTargets()
+            out.e, out.h = ptr(dst["E"]), ptr(dst["H"])
+            out.psi_e, out.psi_h = ptr(dst["psi_e"]), ptr(dst["psi_h"])
+            out.ticket = ptr(solver._ticket)
+            out.steps, out.slots_per_thread, out.threads = 2, 1, 1024
+            plans.append(plan)
+            targets.append(out)
+        n = len(self.solvers)
+        plans = (YeePlan * n)(*plans)
+        targets = (YeeTargets * n)(*targets)
+        from ._lib import lib
+
+        nbytes = int(lib().wfd_batch_pass_workspace_bytes(n))
+        ws = Array.zeros((nbytes + 3) // 4, np.int32, self.device)
+        call("wfd_batch_pass_prepare", plans, targets, n, ptr(ws), nbytes, stream())
+        return plans, targets, ws
+
+    def _pass_launch(self, record, direction):
+        plans, targets, ws = self._pass_tables[(record, direction)]
+        call("wfd_batch_pass", plans, targets, len(self.solvers), ptr(ws), stream())
+
+    def pass_applies(self):
+        return all(s.single_pass_applies() for s in self.solvers)
+
+    def tune_pass(self, steps=8):
+        """Microseconds per step of the batched two-step pass and of the
+        batched fused step (quiet tables, HIP events); keeps the pass when it
+        is faster.  Returns (pass, fused)."""
+        for s in self.solvers:
+            single = s.single_pass
+            s.use_single_pass(0, 1, 2, 1024)  # allocates the second buffer set and ticket
+            s.single_pass = single
+        self._pass_tables = {(r, d): self._pass_table(r, d) for r in (True, False) for d in (0, 1)}
+        plans, ws = self._table(False)
+        timings = []
+        for run in (lambda: [self._pass_launch(False, 0), self._pass_launch(False, 1)],
+                    lambda: [call("wfd_batch_step", plans, len(self.solvers), ptr(ws), stream())
+                             for _ in range(2)]):
+            run()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(steps // 2):
+                run()
+            b.record()
+            torch.cuda.synchronize(self.device)
+            timings.append(a.elapsed_time(b) * 1e3 / (2 * (steps // 2)))
+        self.reset()
+        self.pass_us = {"pass": round(timings[0], 1), "fused": round(timings[1], 1)}
+        return timings[0], timings[1]
 
     def step(self, record=True):
         """One step of every design (eager); staged detectors replay each step."""
@@ -69,7 +137,12 @@ class Batch:
                 done = 0
                 while done < length:
                     m = min(self.graph_steps, length - done)
-                    for _ in range(m):
+                    # two-step passes in pairs (back in the first sets), the
+                    # rest as fused steps
+                    passes = 2 * (m // 4) if self.use_pass else 0
+                    for k in range(passes):
+                        self._pass_launch(record, k % 2)
+                    for _ in range(m - 2 * passes):
                         call("wfd_batch_step", plans, len(self.solvers), ptr(ws), stream())
                     if record:
                         for solver in self.solvers:
@@ -115,6 +188,9 @@ class Batch:
             s.tune()
             s.reset()
         self.x_chunk_us = self.tune_x_chunk()
+        if self.pass_mode is not False and self.pass_applies():
+            pass_us, fused_us = self.tune_pass()
+            self.use_pass = self.pass_mode is True or pass_us < fused_us
         self._destroy_graphs()
         self._tables = {True: self._table(True), False: self._table(False)}
         for length in {self.graph_steps, self.steps % self.graph_steps} - {0}:

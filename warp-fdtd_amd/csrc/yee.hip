@@ -953,13 +953,13 @@ __device__ __forceinline__ void record_curl_at(const PassArgs& pa, int t, int x,
 }
 
 template <int T, int J>
-__global__ __launch_bounds__(T) void k_yee_pass2(PassArgs pa) {
+__device__ __forceinline__ void pass2_body(const PassArgs& pa, int bid, int nblocks) {
   constexpr int SLOTS = J * T;
   __shared__ float le1[2][3][SLOTS];  // E1 of planes x - 1 and x
   __shared__ float lh1[3][SLOTS];     // H1 of plane x - 1
   __shared__ float le2[2][3][SLOTS];  // E2 of planes x - 2 and x - 1
   const YeeArgs& a = pa.a;
-  const int logical = (blockIdx.x & 7) * pa.per + (blockIdx.x >> 3);
+  const int logical = (bid & 7) * pa.per + (bid >> 3);
   const int t = a.counters[0];
   if (logical < pa.total) {
     const int nx = a.nx, ny = a.ny, nz = a.nz, P = a.P, rows = a.rows;
@@ -1205,12 +1205,28 @@ __global__ __launch_bounds__(T) void k_yee_pass2(PassArgs pa) {
   }
   if (threadIdx.x == 0) {
     const int done = atomicAdd(pa.ticket, 1);
-    if (done == (int)gridDim.x - 1) {
+    if (done == nblocks - 1) {
       a.counters[0] = t + 2;
       a.counters[1] = t + 1;
       *pa.ticket = 0;
     }
   }
+}
+
+template <int T, int J>
+__global__ __launch_bounds__(T) void k_yee_pass2(PassArgs pa) {
+  pass2_body<T, J>(pa, blockIdx.x, gridDim.x);
+}
+
+// Batched two-step pass: the grid concatenates the (XCD-striped, padded to
+// multiples of 8) block ranges of `count` passes, first[d] the first block of
+// pass d; each design's arithmetic, ticket and counters are its own pass's,
+// so its result is bitwise that of its own wfd_yee_pass.
+template <int T, int J>
+__global__ __launch_bounds__(T) void k_batch_pass2(const PassArgs* __restrict__ table,
+                                                   const int* __restrict__ first, int count) {
+  const int d = batch_member(first, count, blockIdx.x);
+  pass2_body<T, J>(table[d], blockIdx.x - first[d], first[d + 1] - first[d]);
 }
 
 // Validate a plan and fill the fused step's arguments (shared by the single
@@ -1315,26 +1331,85 @@ int launch_update(float* f, const float* g, const float* inv_eps, float* psi,
 // x planes per workgroup for a pass shape: a workgroup marches its chunk
 // with a barrier per plane, so the pass takes about (rounds of co-resident
 // workgroups per XCD) x (planes a march visits: the chunk plus one halo plane
-// per step).  The count minimising that, from the kernel's occupancy.
-int pass_planes(const void* kernel, int threads, int tiles, int nx, int steps) {
+// per step).  The count minimising that, from the kernel's occupancy, for
+// `count` grids launched together (tiles[d] tiles per plane, nx[d] planes).
+int pass_planes(const void* kernel, int threads, const int* tiles, const int* nx, int count,
+                int steps) {
   int dev = 0, cus = 0, occ = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
       hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, threads, 0) != hipSuccess)
     return 8;
   const int64_t per_xcd = (int64_t)max(cus, 8) / 8 * max(occ, 1);
+  int longest = 1;
+  for (int d = 0; d < count; ++d) longest = max(longest, nx[d]);
   int best = 1;
   int64_t best_cost = INT64_MAX;
-  for (int xc = 1; xc <= min(64, nx); ++xc) {
-    const int64_t wgs = (int64_t)tiles * ((nx + xc - 1) / xc);
-    const int64_t rounds = ((wgs + 7) / 8 + per_xcd - 1) / per_xcd;
-    const int64_t cost = rounds * (xc + steps);
+  for (int xc = 1; xc <= min(64, longest); ++xc) {
+    int64_t blocks = 0;  // per XCD
+    for (int d = 0; d < count; ++d)
+      blocks += ((int64_t)tiles[d] * ((nx[d] + xc - 1) / xc) + 7) / 8;
+    const int64_t cost = (blocks + per_xcd - 1) / per_xcd * (xc + steps);
     if (cost < best_cost) {
       best_cost = cost;
       best = xc;
     }
   }
   return best;
+}
+
+// Validate a pass (plan = the step's start, out = its targets) and fill its
+// arguments but for the x chunking; the shape it launches in steps,
+// per_thread and threads.
+int pass_setup(const wfd_yee_plan* p, const wfd_yee_targets* out, PassArgs& pa, int& steps,
+               int& per_thread, int& threads) {
+  if (int status = build_args(p, pa.a, "yee_pass")) return status;
+  WFD_REQUIRE(out && out->e && out->h && out->psi_e && out->psi_h && out->ticket,
+              "yee_pass: null target or ticket pointer");
+  // psi (6 components) is addressed through 32-bit buffer offsets
+  WFD_REQUIRE((int64_t)24 * pa.a.n <= kFieldBytes,
+              "yee_pass: grids above %d cells step with wfd_yee_step", (int)(kFieldBytes / 24));
+  WFD_REQUIRE(out->e != p->e && out->h != p->h && out->psi_e != p->psi_e &&
+                  out->psi_h != p->psi_h,
+              "yee_pass: targets must be buffers other than the plan's");
+  steps = out->steps ? out->steps : 1;
+  WFD_REQUIRE(steps == 1 || steps == 2, "yee_pass: steps must be 1 or 2");
+  // one step: 512 or 1024 threads x 1 offset, 256 threads x 2 or 4; two
+  // steps: 512 threads x 1 or 2, 1024 threads x 1
+  per_thread = out->slots_per_thread ? out->slots_per_thread : (steps == 1 ? 4 : 2);
+  WFD_REQUIRE(steps == 1 ? (per_thread == 1 || per_thread == 2 || per_thread == 4)
+                         : (per_thread == 1 || per_thread == 2),
+              "yee_pass: slots_per_thread must be 1, 2 or 4 (one step), 1 or 2 (two steps)");
+  threads = out->threads ? out->threads : (steps == 2 || per_thread == 1) ? 2 * kBlock : kBlock;
+  WFD_REQUIRE(threads == 4 * kBlock ? per_thread == 1
+                                    : threads == ((steps == 2 || per_thread == 1) ? 2 * kBlock
+                                                                                  : kBlock),
+              "yee_pass: %d threads with %d slots per thread is not a pass shape", threads,
+              per_thread);
+  const int slots = per_thread * threads;
+  const int halo = steps == 1 ? p->nz : 3 * p->nz;
+  WFD_REQUIRE(halo < slots, "yee_pass: nz = %d leaves no tile in %d slots per workgroup", p->nz,
+              slots);
+  WFD_REQUIRE(out->planes >= 0 && out->planes <= 64, "yee_pass: planes must be 0..64");
+  pa.e_out = out->e;
+  pa.h_out = out->h;
+  pa.pe_out = out->psi_e;
+  pa.ph_out = out->psi_h;
+  pa.ticket = out->ticket;
+  pa.hist = out->hist;
+  pa.hist_steps = out->hist_steps > 0 ? out->hist_steps : 1;
+  for (int k = 0; k < 6; ++k) pa.hbox[k] = out->hist_box[k];
+  if (out->hist) {
+    WFD_REQUIRE(steps == 2, "yee_pass: the curl history is recorded by two-step passes only");
+    WFD_REQUIRE(out->hist_steps > 0, "yee_pass: hist_steps must be positive");
+    const int* b = out->hist_box;
+    WFD_REQUIRE(b[0] >= 0 && b[1] >= 0 && b[2] >= 0 && b[3] >= 0 && b[4] >= 0 && b[5] >= 0 &&
+                    b[0] + b[3] <= p->nx && b[1] + b[4] <= p->ny && b[2] + b[5] <= p->nz,
+                "yee_pass: history box outside the grid");
+  }
+  pa.W = slots - halo;
+  pa.tiles = (pa.a.P + pa.W - 1) / pa.W;
+  return WFD_OK;
 }
 
 }  // namespace
@@ -1397,53 +1472,8 @@ int wfd_yee_step(const wfd_yee_plan* p, void* stream) {
 
 int wfd_yee_pass(const wfd_yee_plan* p, const wfd_yee_targets* out, void* stream) {
   PassArgs pa;
-  if (int status = build_args(p, pa.a, "yee_pass")) return status;
-  WFD_REQUIRE(out && out->e && out->h && out->psi_e && out->psi_h && out->ticket,
-              "yee_pass: null target or ticket pointer");
-  // psi (6 components) is addressed through 32-bit buffer offsets
-  WFD_REQUIRE((int64_t)24 * pa.a.n <= kFieldBytes,
-              "yee_pass: grids above %d cells step with wfd_yee_step", (int)(kFieldBytes / 24));
-  WFD_REQUIRE(out->e != p->e && out->h != p->h && out->psi_e != p->psi_e &&
-                  out->psi_h != p->psi_h,
-              "yee_pass: targets must be buffers other than the plan's");
-  const int steps = out->steps ? out->steps : 1;
-  WFD_REQUIRE(steps == 1 || steps == 2, "yee_pass: steps must be 1 or 2");
-  // one step: 512 or 1024 threads x 1 offset, 256 threads x 2 or 4; two
-  // steps: 512 threads x 1 or 2, 1024 threads x 1
-  const int per_thread = out->slots_per_thread ? out->slots_per_thread : (steps == 1 ? 4 : 2);
-  WFD_REQUIRE(steps == 1 ? (per_thread == 1 || per_thread == 2 || per_thread == 4)
-                         : (per_thread == 1 || per_thread == 2),
-              "yee_pass: slots_per_thread must be 1, 2 or 4 (one step), 1 or 2 (two steps)");
-  const int threads = out->threads ? out->threads
-                                   : (steps == 2 || per_thread == 1) ? 2 * kBlock : kBlock;
-  WFD_REQUIRE(threads == 4 * kBlock ? per_thread == 1
-                                    : threads == ((steps == 2 || per_thread == 1) ? 2 * kBlock
-                                                                                  : kBlock),
-              "yee_pass: %d threads with %d slots per thread is not a pass shape", threads,
-              per_thread);
-  const int slots = per_thread * threads;
-  const int halo = steps == 1 ? p->nz : 3 * p->nz;
-  WFD_REQUIRE(halo < slots, "yee_pass: nz = %d leaves no tile in %d slots per workgroup", p->nz,
-              slots);
-  WFD_REQUIRE(out->planes >= 0 && out->planes <= 64, "yee_pass: planes must be 0..64");
-  pa.e_out = out->e;
-  pa.h_out = out->h;
-  pa.pe_out = out->psi_e;
-  pa.ph_out = out->psi_h;
-  pa.ticket = out->ticket;
-  pa.hist = out->hist;
-  pa.hist_steps = out->hist_steps > 0 ? out->hist_steps : 1;
-  for (int k = 0; k < 6; ++k) pa.hbox[k] = out->hist_box[k];
-  if (out->hist) {
-    WFD_REQUIRE(steps == 2, "yee_pass: the curl history is recorded by two-step passes only");
-    WFD_REQUIRE(out->hist_steps > 0, "yee_pass: hist_steps must be positive");
-    const int* b = out->hist_box;
-    WFD_REQUIRE(b[0] >= 0 && b[1] >= 0 && b[2] >= 0 && b[3] >= 0 && b[4] >= 0 && b[5] >= 0 &&
-                    b[0] + b[3] <= p->nx && b[1] + b[4] <= p->ny && b[2] + b[5] <= p->nz,
-                "yee_pass: history box outside the grid");
-  }
-  pa.W = slots - halo;
-  pa.tiles = (pa.a.P + pa.W - 1) / pa.W;
+  int steps, per_thread, threads;
+  if (int status = pass_setup(p, out, pa, steps, per_thread, threads)) return status;
   int xc = out->planes;
   if (xc == 0) {
     const void* kernel =
@@ -1454,7 +1484,7 @@ int wfd_yee_pass(const wfd_yee_plan* p, const wfd_yee_targets* out, void* stream
         : per_thread == 1       ? (const void*)&k_yee_pass<2 * kBlock, 1>
         : per_thread == 2       ? (const void*)&k_yee_pass<kBlock, 2>
                                 : (const void*)&k_yee_pass<kBlock, 4>;
-    xc = pass_planes(kernel, threads, pa.tiles, p->nx, steps);
+    xc = pass_planes(kernel, threads, &pa.tiles, &p->nx, 1, steps);
   }
   pa.xc = xc;
   const int chunks = (p->nx + xc - 1) / xc;
@@ -1558,6 +1588,87 @@ int wfd_batch_step(const wfd_yee_plan* plans, int count, const void* workspace, 
     }
   }
   WFD_LAUNCH_CHECK("batch_step");
+  return WFD_OK;
+}
+
+
+// ------------------------------------------------------------ batched pass
+// Two-step passes of many grids in one launch (k_batch_pass2<1024, 1>).
+namespace {
+int batch_pass_args(const wfd_yee_plan* plans, const wfd_yee_targets* targets, int count,
+                    PassArgs* table, int* first) {
+  WFD_REQUIRE(plans && targets, "batch_pass: null argument");
+  WFD_REQUIRE(count >= 1 && count <= WFD_MAX_BATCH, "batch_pass: 1..%d passes", WFD_MAX_BATCH);
+  int tiles[WFD_MAX_BATCH], nx[WFD_MAX_BATCH];
+  for (int d = 0; d < count; ++d) {
+    wfd_yee_targets out = targets[d];
+    WFD_REQUIRE(out.steps == 2 && (out.slots_per_thread == 0 || out.slots_per_thread == 1) &&
+                    (out.threads == 0 || out.threads == 4 * kBlock),
+                "batch_pass: pass %d is not a two-step pass of 1024 x 1 slots", d);
+    WFD_REQUIRE(out.planes == targets[0].planes, "batch_pass: passes differ in planes");
+    out.slots_per_thread = 1;
+    out.threads = 4 * kBlock;
+    int steps, per_thread, threads;
+    if (int status = pass_setup(plans + d, &out, table[d], steps, per_thread, threads))
+      return status;
+    for (int e = 0; e < d; ++e)
+      WFD_REQUIRE(targets[e].ticket != out.ticket && plans[e].counters != plans[d].counters,
+                  "batch_pass: passes %d and %d share a ticket or counters", e, d);
+    tiles[d] = table[d].tiles;
+    nx[d] = plans[d].nx;
+  }
+  int xc = targets[0].planes;
+  if (xc == 0)
+    xc = pass_planes((const void*)&k_batch_pass2<4 * kBlock, 1>, 4 * kBlock, tiles, nx, count,
+                     2);
+  WFD_REQUIRE(xc >= 1 && xc <= 64, "batch_pass: planes must be 0..64");
+  int at = 0;
+  for (int d = 0; d < count; ++d) {
+    PassArgs& pa = table[d];
+    pa.xc = xc;
+    pa.total = pa.tiles * ((nx[d] + xc - 1) / xc);
+    pa.per = (pa.total + 7) / 8;
+    first[d] = at;
+    at += pa.per * 8;
+  }
+  first[count] = at;
+  return WFD_OK;
+}
+}  // namespace
+
+int64_t wfd_batch_pass_workspace_bytes(int count) {
+  if (count < 1 || count > WFD_MAX_BATCH) return -1;
+  return (int64_t)count * sizeof(PassArgs) + (int64_t)(count + 1) * sizeof(int);
+}
+
+int wfd_batch_pass_prepare(const wfd_yee_plan* plans, const wfd_yee_targets* targets, int count,
+                           void* workspace, int64_t workspace_bytes, void* stream) {
+  WFD_REQUIRE(workspace, "batch_pass_prepare: null workspace");
+  WFD_REQUIRE(workspace_bytes >= wfd_batch_pass_workspace_bytes(count),
+              "batch_pass_prepare: workspace smaller than wfd_batch_pass_workspace_bytes");
+  PassArgs table[WFD_MAX_BATCH];
+  int first[WFD_MAX_BATCH + 1];
+  if (int status = batch_pass_args(plans, targets, count, table, first)) return status;
+  hipStream_t s = as_stream(stream);
+  char* ws = static_cast<char*>(workspace);
+  WFD_HIP(hipMemcpyAsync(ws, table, sizeof(PassArgs) * count, hipMemcpyHostToDevice, s));
+  WFD_HIP(hipMemcpyAsync(ws + sizeof(PassArgs) * count, first, sizeof(int) * (count + 1),
+                         hipMemcpyHostToDevice, s));
+  WFD_HIP(hipStreamSynchronize(s));
+  return WFD_OK;
+}
+
+int wfd_batch_pass(const wfd_yee_plan* plans, const wfd_yee_targets* targets, int count,
+                   const void* workspace, void* stream) {
+  WFD_REQUIRE(workspace, "batch_pass: null workspace");
+  PassArgs table[WFD_MAX_BATCH];
+  int first[WFD_MAX_BATCH + 1];
+  if (int status = batch_pass_args(plans, targets, count, table, first)) return status;
+  const PassArgs* t = static_cast<const PassArgs*>(workspace);
+  const int* f = reinterpret_cast<const int*>(t + count);
+  hipLaunchKernelGGL((k_batch_pass2<4 * kBlock, 1>), dim3(first[count]), dim3(4 * kBlock), 0,
+                     as_stream(stream), t, f, count);
+  WFD_LAUNCH_CHECK("batch_pass");
   return WFD_OK;
 }
 
