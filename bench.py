@@ -350,6 +350,69 @@ def bench_batch(args):
            "batched_us_per_step_pass_vs_fused": bt.batch.pass_us})
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
+
+
+def bench_fno(args):
+    """Training throughput of the config-5 FNO (100x100 input, width 32, 12
+    modes, 4 layers, 9 outputs, batch 16) on synthetic labels: Adam steps of
+    forward + backward on the BF16 MFMA GEMMs.  The roofline object counts
+    the GEMMs' multiply-adds (2 flops each) per step over the whole step's
+    time, so it bounds their rate from below."""
+    import torch
+
+    import __graft_entry__
+
+    pkg = __graft_entry__.package()
+    sg, fno = pkg.surrogate, pkg.fno
+    side, batch = 100, 16
+    net = fno.FNO(side, batch=batch, outputs=9, seed=0)
+    rng = np.random.default_rng(0)
+    n = 16 * batch
+    rhos = rng.uniform(0, 1, (n, side, side))
+    T = rng.uniform(0.0, 0.1, (n, 9))
+    G = rng.normal(0.0, 1e-4, (n, side, side))
+    fit = sg._Fitter(net, sg.Normalizer(T, G), 1.0)
+    x, tn, gn = fit.tensors(rhos, T, G)
+    batches = [torch.arange(k * batch, (k + 1) * batch, device=net.device) for k in range(16)]
+    flops = [0]
+    plain = fno.FNO.gemm
+
+    def counted(a, a16, sa, b, b16, sb, c, sc, bt, m, nn, k, *rest, **kw):
+        flops[0] += 2 * int(bt) * int(m) * int(nn) * int(k)
+        return plain(a, a16, sa, b, b16, sb, c, sc, bt, m, nn, k, *rest, **kw)
+
+    fno.FNO.gemm = staticmethod(counted)
+    try:
+        fit.step(x, tn, gn, batches[0], 1e-4)
+    finally:
+        fno.FNO.gemm = staticmethod(plain)
+    per_step = flops[0]
+    for k in range(args.warmup):
+        fit.step(x, tn, gn, batches[k % 16], 1e-4)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    steps = max(args.steps, 1) * 10
+    a.record()
+    for k in range(steps):
+        fit.step(x, tn, gn, batches[k % 16], 1e-4)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    achieved = per_step / (ms * 1e-3) / 1e12
+    _line("FNO training samples/s (100x100 density, width 32, 12 modes, BF16 MFMA)",
+          round(batch / (ms * 1e-3), 1), "samples/s", steps, args.warmup, round(ms, 4),
+          {"workload": "config-5 FNO, batch 16, Adam step = forward + backward + update on "
+                       "synthetic densities and labels", "batch": batch,
+           "gemm_flops_per_step": per_step},
+          dtype="bf16",
+          roofline={"bound": "mfma", "achieved": round(achieved, 3),
+                    "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 6), "traffic": None,
+                    "kernel": "wfd_gemm_bf16 (k_gemm) multiply-adds over the whole step"},
+          cpu_baseline=None)
+
+
 def bench_surrogate_label(args):
     """Training labels of the surrogate: designs first .. first + count - 1 of
     the seeded blurred-random family on the 200x200x80 bend at 20 nm (4000
@@ -543,7 +606,7 @@ def main():
                    help="only run 8 device-to-device copies of this many MiB (the reference "
                         "traffic for calibrating HBM counters) and exit")
     p.add_argument("--workload", choices=("waveguide", "slab", "bend-adjoint", "batch",
-                                           "surrogate", "surrogate-label"),
+                                           "surrogate", "surrogate-label", "fno"),
                    default="waveguide",
                    help="waveguide: the headline Yee-step metric; the others measure the "
                         "adjoint, batched-design and surrogate workloads")
@@ -565,7 +628,7 @@ def main():
     args = p.parse_args()
     if args.workload != "waveguide" and args.gpus == 1:
         {"slab": bench_slab, "bend-adjoint": bench_adjoint, "batch": bench_batch,
-         "surrogate": bench_surrogate,
+         "surrogate": bench_surrogate, "fno": bench_fno,
          "surrogate-label": bench_surrogate_label}[args.workload](args)
         return
     if args.copy_mib:

@@ -476,8 +476,12 @@ int wfd_act_back(const float* dout, const float* pre, float* dz, long long total
 long long wfd_colsum_scratch(long long rows, int n);
 int wfd_colsum(const float* x, long long rows, int n, float* out, float* scratch,
                long long scratch_floats, float beta, void* stream);
-/* y[b, j] = mean_i x[b, i, j] and its transpose (dx = dy / p + beta * dx). */
-int wfd_pool_mean(const float* x, int batch, int p, int w, float* y, void* stream);
+/* y[b, j] = mean_i x[b, i, j] (runs of 512 pixels summed in parallel, then
+ * in order; scratch holds wfd_pool_mean_scratch(batch, p, w) floats) and its
+ * transpose (dx = dy / p + beta * dx). */
+long long wfd_pool_mean_scratch(int batch, int p, int w);
+int wfd_pool_mean(const float* x, int batch, int p, int w, float* y, float* scratch,
+                  long long scratch_floats, void* stream);
 int wfd_pool_mean_back(const float* dy, int batch, int p, int w, float* dx, float beta,
                        void* stream);
 /* d = scale * (pred - target); partial[k] = sum of squared errors of block k

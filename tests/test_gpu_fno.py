@@ -97,7 +97,11 @@ def test_elementwise_and_reductions(pkg, gpu):
     x = torch.randn(4, 250, 9, generator=g)
     pooled = torch.empty(4, 9).cuda()
     x_d = x.cuda()
-    lib.call("wfd_pool_mean", P(x_d.data_ptr()), 4, 250, 9, P(pooled.data_ptr()), s)
+    pscratch = torch.empty(lib.lib().wfd_pool_mean_scratch(4, 1250, 9)).cuda()
+    x = torch.randn(4, 1250, 9, generator=g)  # (three runs of pixels and a ragged one)
+    x_d = x.cuda()
+    lib.call("wfd_pool_mean", P(x_d.data_ptr()), 4, 1250, 9, P(pooled.data_ptr()),
+             P(pscratch.data_ptr()), pscratch.numel(), s)
     torch.cuda.synchronize()
     assert (pooled.cpu().double() - x.double().mean(1)).abs().max() < 1e-6
     pred, target = torch.randn(1000, generator=g), torch.randn(1000, generator=g)

@@ -175,7 +175,8 @@ SIGNATURES = {
     "wfd_act_back": [P, P, P, L, I, P],
     "wfd_colsum_scratch": [L, I],
     "wfd_colsum": [P, L, I, P, P, L, F, P],
-    "wfd_pool_mean": [P, I, I, I, P, P],
+    "wfd_pool_mean_scratch": [I, I, I],
+    "wfd_pool_mean": [P, I, I, I, P, P, L, P],
     "wfd_pool_mean_back": [P, I, I, I, P, F, P],
     "wfd_mse_partials": [L],
     "wfd_mse": [P, P, L, F, P, P, P],
@@ -202,6 +203,7 @@ def lib():
                           "wfd_batch_workspace_bytes": ctypes.c_int64,
                           "wfd_batch_pass_workspace_bytes": ctypes.c_int64,
                           "wfd_colsum_scratch": ctypes.c_int64,
+                          "wfd_pool_mean_scratch": ctypes.c_int64,
                           "wfd_mse_partials": ctypes.c_int64}.get(name, ctypes.c_int)
         _lib = handle
     return _lib

@@ -153,23 +153,20 @@ constexpr int kRowsPerBlock = 1024;
 
 __global__ void k_colsum_partial(const float* __restrict__ x, long long rows, int n,
                                  float* __restrict__ partial) {
+  // block (row range, group of 32 columns): 8 row lanes x 32 columns
   __shared__ float red[kBlock];
   const long long r0 = (long long)blockIdx.x * kRowsPerBlock;
   const long long r1 = min(rows, r0 + kRowsPerBlock);
-  for (int c0 = 0; c0 < n; c0 += kBlock / 8) {
-    // 8 row lanes x 32 columns per pass
-    const int c = c0 + (threadIdx.x & 31), lanes = 8, rl = threadIdx.x >> 5;
-    float s = 0.f;
-    if (c < n)
-      for (long long r = r0 + rl; r < r1; r += lanes) s += x[r * n + c];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    if (rl == 0 && c < n) {
-      float t = 0.f;
-      for (int q = 0; q < lanes; ++q) t += red[q * 32 + (threadIdx.x & 31)];
-      partial[(long long)blockIdx.x * n + c] = t;
-    }
-    __syncthreads();
+  const int c = blockIdx.y * 32 + (threadIdx.x & 31), lanes = 8, rl = threadIdx.x >> 5;
+  float s = 0.f;
+  if (c < n)
+    for (long long r = r0 + rl; r < r1; r += lanes) s += x[r * n + c];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (rl == 0 && c < n) {
+    float t = 0.f;
+    for (int q = 0; q < lanes; ++q) t += red[q * 32 + (threadIdx.x & 31)];
+    partial[(long long)blockIdx.x * n + c] = t;
   }
 }
 
@@ -182,20 +179,36 @@ __global__ void k_colsum_final(const float* __restrict__ partial, int parts, int
   out[c] = beta != 0.f ? s + beta * out[c] : s;
 }
 
-// mean over the p pixels of each batch member: x (b, p, w) -> y (b, w)
-__global__ void k_pool_mean(const float* __restrict__ x, int p, int w, float* __restrict__ y) {
+// mean over the p pixels of each batch member: x (b, p, w) -> y (b, w).
+// Pass 1: block (member, group of 32 channels, run of kPoolRows pixels) sums
+// its run (8 pixel lanes x 32 channels); pass 2 adds the runs in order.
+constexpr int kPoolRows = 512;
+
+__global__ void k_pool_partial(const float* __restrict__ x, int p, int w,
+                               float* __restrict__ partial) {
   __shared__ float red[kBlock];
   const int b = blockIdx.x, c = blockIdx.y * 32 + (threadIdx.x & 31), rl = threadIdx.x >> 5;
+  const int i0 = blockIdx.z * kPoolRows, i1 = min(p, i0 + kPoolRows);
   float s = 0.f;
   if (c < w)
-    for (int i = rl; i < p; i += 8) s += x[((long long)b * p + i) * w + c];
+    for (int i = i0 + rl; i < i1; i += 8) s += x[((long long)b * p + i) * w + c];
   red[threadIdx.x] = s;
   __syncthreads();
   if (rl == 0 && c < w) {
     float t = 0.f;
     for (int q = 0; q < 8; ++q) t += red[q * 32 + (threadIdx.x & 31)];
-    y[(long long)b * w + c] = t / (float)p;
+    partial[((long long)b * gridDim.z + blockIdx.z) * w + c] = t;
   }
+}
+
+__global__ void k_pool_final(const float* __restrict__ partial, int b, int runs, int w, int p,
+                             float* __restrict__ y) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= b * w) return;
+  const int m = i / w, c = i % w;
+  float s = 0.f;
+  for (int k = 0; k < runs; ++k) s += partial[((long long)m * runs + k) * w + c];
+  y[i] = s / (float)p;
 }
 
 __global__ void k_pool_mean_back(const float* __restrict__ dy, long long total, int p, int w,
@@ -330,18 +343,30 @@ int wfd_colsum(const float* x, long long rows, int n, float* out, float* scratch
   WFD_REQUIRE(scratch_floats >= parts * n, "colsum: scratch smaller than wfd_colsum_scratch");
   WFD_REQUIRE(parts <= 2147483647, "colsum: too many rows");
   hipStream_t s = as_stream(stream);
-  hipLaunchKernelGGL(k_colsum_partial, dim3((unsigned)parts), dim3(kBlock), 0, s, x, rows, n,
-                     scratch);
+  hipLaunchKernelGGL(k_colsum_partial, dim3((unsigned)parts, (n + 31) / 32), dim3(kBlock), 0, s,
+                     x, rows, n, scratch);
   hipLaunchKernelGGL(k_colsum_final, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, scratch,
                      (int)parts, n, out, beta);
   WFD_LAUNCH_CHECK("colsum");
   return WFD_OK;
 }
 
-int wfd_pool_mean(const float* x, int batch, int p, int w, float* y, void* stream) {
-  WFD_REQUIRE(x && y && batch >= 1 && p >= 1 && w >= 1, "pool_mean: bad argument");
-  hipLaunchKernelGGL(k_pool_mean, dim3(batch, (w + 31) / 32), dim3(kBlock), 0, as_stream(stream),
-                     x, p, w, y);
+long long wfd_pool_mean_scratch(int batch, int p, int w) {
+  if (batch < 1 || p < 1 || w < 1) return -1;
+  return (long long)batch * ((p + kPoolRows - 1) / kPoolRows) * w;
+}
+
+int wfd_pool_mean(const float* x, int batch, int p, int w, float* y, float* scratch,
+                  long long scratch_floats, void* stream) {
+  WFD_REQUIRE(x && y && scratch && batch >= 1 && p >= 1 && w >= 1, "pool_mean: bad argument");
+  const int runs = (p + kPoolRows - 1) / kPoolRows;
+  WFD_REQUIRE(scratch_floats >= (long long)batch * runs * w,
+              "pool_mean: scratch smaller than wfd_pool_mean_scratch");
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(k_pool_partial, dim3(batch, (w + 31) / 32, runs), dim3(kBlock), 0, s, x, p,
+                     w, scratch);
+  hipLaunchKernelGGL(k_pool_final, dim3((batch * w + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                     scratch, batch, runs, w, p, y);
   WFD_LAUNCH_CHECK("pool_mean");
   return WFD_OK;
 }

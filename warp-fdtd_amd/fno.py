@@ -118,6 +118,12 @@ def init_parameters(layout, width, modes, seed=0):
     return flat
 
 
+def _splits(rows, most=32):
+    """The largest divisor of ``rows`` up to ``most``: row runs per sample of
+    a weight-gradient product."""
+    return max(d for d in range(1, most + 1) if rows % d == 0)
+
+
 class FNO:
     """The surrogate on libwfdtd kernels (training and inference)."""
 
@@ -167,10 +173,11 @@ class FNO:
         self.d_thid = torch.zeros(B, self.H, **f32)
         self.d_pool = torch.zeros(B, W, **f32)
         widest = max(W * W, self.H * max(W, outputs), 3 * W)
-        self.partials = torch.zeros(B, widest, **f32)
+        self.partials = torch.zeros(B * _splits(S * S), widest, **f32)
         rows = B * S * S
         need = max(lib().wfd_colsum_scratch(rows, max(self.H, W)),
-                   lib().wfd_colsum_scratch(B, widest))
+                   lib().wfd_colsum_scratch(B * _splits(S * S), widest),
+                   lib().wfd_pool_mean_scratch(B, S * S, W))
         self.scratch = torch.zeros(int(need), **f32)
 
     # ------------------------------------------------------------ helpers
@@ -281,7 +288,8 @@ class FNO:
         call("wfd_bias_act", _p(self.g_out), _p(self.p("g2_b")), None, _p(self.g_out), rows, 1,
              0, stream())
         # transmission head
-        call("wfd_pool_mean", _p(top), B, S * S, W, _p(self.pool), stream())
+        call("wfd_pool_mean", _p(top), B, S * S, W, _p(self.pool), _p(self.scratch),
+             self.scratch.numel(), stream())
         self.linear(self.pool, B, W, "t1_w", H, self.t_pre)
         call("wfd_bias_act", _p(self.t_pre), _p(self.p("t1_b")), _p(self.t_pre),
              _p(self.t_hid), B, H, 1, stream())
@@ -292,12 +300,18 @@ class FNO:
 
     # ------------------------------------------------------------ backward
     def _wgrad(self, x, rows_per, k, dy, n, name):
-        """grad[name] (k, n) = sum_b x_b^T (k x rows) . dy_b (rows x n)."""
+        """grad[name] (k, n) = sum_b x_b^T (k x rows) . dy_b (rows x n).  Each
+        sample's rows are split into ``splits`` equal runs (one GEMM batch
+        entry and one partial product each, so the small k x n products of a
+        10,000-row sample spread over hundreds of workgroups instead of one
+        per sample), and the partials are summed in a fixed order."""
         B = self.B
-        part = self.partials.view(-1)[:B * k * n].view(B, k * n)
-        self.gemm(x, False, (rows_per * k, 1, k), dy, False, (rows_per * n, n, 1), part,
-                  (k * n, n, 1), B, k, n, rows_per)
-        self.colsum(part, B, k * n, self.gview(name).view(-1), beta=0.0)
+        splits = _splits(rows_per)
+        R = rows_per // splits
+        part = self.partials.view(-1)[:B * splits * k * n].view(B * splits, k * n)
+        self.gemm(x, False, (R * k, 1, k), dy, False, (R * n, n, 1), part,
+                  (k * n, n, 1), B * splits, k, n, R)
+        self.colsum(part, B * splits, k * n, self.gview(name).view(-1), beta=0.0)
 
     def backward(self, d_t, d_g):
         """Accumulate parameter gradients of sum(d_t * T) + sum(d_g * g)
