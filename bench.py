@@ -225,7 +225,8 @@ def bench_adjoint(args):
     if not (path / "manifest.json").exists():
         pkg.cases.bend_case(path, rho, shape=(200, 200, 80), steps=4000, pml=10,
                             design=(100, 100), wavelength=1.55e-6)
-    problem = pkg.design.TransmissionProblem(path, checkpoint_every=64)
+    every = args.checkpoint_every or None  # 0: the whole curl history, no recompute
+    problem = pkg.design.TransmissionProblem(path, checkpoint_every=every)
     problem.value_and_grad(rho)  # compiles the graphs
     t0 = time.perf_counter()
     T, grad = problem.value_and_grad(rho)
@@ -248,7 +249,9 @@ def bench_adjoint(args):
               "max_rel": float(np.max(np.abs(a - f) / np.abs(f))), "fd": f.tolist(),
               "adjoint": a.tolist(), "precision": "float32 forward runs on the GPU"}
     n = problem.adjoint.n
-    _line("adjoint gradients/s (200x200x80 bend, 100x100 voxels, 4000 steps, checkpoints/64)",
+    memory = (f"checkpoints/{every}" if every else
+              "full curl history in HBM, no recompute")
+    _line(f"adjoint gradients/s (200x200x80 bend, 100x100 voxels, 4000 steps, {memory})",
           round(1.0 / seconds, 4), "gradients/s", 4000, 1, round(seconds * 1e3 / 4000, 5),
           {"workload": "90-degree Si bend 200x200x80 @ 20 nm, rho~U[0,1) seed 0, "
                        "eps = 2.085 + rho*9.995 through 11 z-cells, TE0 T at 1.55 um",
@@ -611,6 +614,9 @@ def main():
                    help="waveguide: the headline Yee-step metric; the others measure the "
                         "adjoint, batched-design and surrogate workloads")
     p.add_argument("--fd", action="store_true", help="bend-adjoint: central-difference check")
+    p.add_argument("--checkpoint-every", type=int, default=64,
+                   help="bend-adjoint: steps between field checkpoints (0: keep the whole "
+                        "design-box curl history instead and skip the recompute)")
     p.add_argument("--labels", default="",
                    help="surrogate: directories of label files (comma-separated)")
     p.add_argument("--held-out", type=int, default=64,
