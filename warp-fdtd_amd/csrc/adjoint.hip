@@ -376,14 +376,16 @@ struct InjectPlane {
   int x0, y0, z0, sx, sy, sz, count;
 };
 
-// value of component c at plane cell (x, y, z); 0 outside the plane
+// value of component c at plane cell (x, y, z); 0 outside the plane.  The
+// load is unconditional (a cell outside reads the plane's first cell and the
+// value is dropped), so the stencil's loads issue together.
 __device__ __forceinline__ float plane_value(const InjectPlane& p, int c, int x, int y, int z) {
-  if (x < p.x0 || y < p.y0 || z < p.z0 || x >= p.x0 + p.sx || y >= p.y0 + p.sy ||
-      z >= p.z0 + p.sz)
-    return 0.f;
-  const int i = ((x - p.x0) * p.sy + (y - p.y0)) * p.sz + (z - p.z0);
+  const bool in = x >= p.x0 && y >= p.y0 && z >= p.z0 && x < p.x0 + p.sx &&
+                  y < p.y0 + p.sy && z < p.z0 + p.sz;
+  const int i = in ? ((x - p.x0) * p.sy + (y - p.y0)) * p.sz + (z - p.z0) : 0;
   const float2 s = p.source[c * p.count + i];
-  return s.x * p.w0 - s.y * p.w1;
+  const float v = s.x * p.w0 - s.y * p.w1;
+  return in ? v : 0.f;
 }
 
 // One plane's adjoint source for the reverse step (a wfd_inject_source).
