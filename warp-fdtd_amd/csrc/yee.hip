@@ -614,7 +614,10 @@ struct PassArgs {
 };
 
 template <int T, int J>
-__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(J == 1 ? 8 : 1))) void k_yee_pass(PassArgs pa) {
+// (one offset per thread: at most 64 VGPRs, 8 waves per SIMD; the compiler
+// would otherwise take 65)
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(J == 1 ? 8 : 1)))
+void k_yee_pass(PassArgs pa) {
   constexpr int SLOTS = J * T;
   __shared__ float lds[2][3][SLOTS];  // the new E of planes x - 1 and x
   const YeeArgs& a = pa.a;
