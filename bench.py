@@ -349,7 +349,7 @@ def bench_batch(args):
            "x_chunks": sorted({s.x_chunk for s in bt.batch.solvers}),
            "batched_step_us_by_x_chunk": {k: round(v, 1)
                                           for k, v in bt.batch.x_chunk_us.items()},
-           "batched_two_step_pass": bt.batch.use_pass,
+           "batched_two_step_pass": bt.batch.use_pass, "pass_per_design": bt.batch.pass_each,
            "batched_us_per_step_pass_vs_fused": bt.batch.pass_us})
 
 

@@ -32,7 +32,7 @@ def _solvers(pkg, cases, chunks):
 
 
 @pytest.mark.parametrize("chunks,single_pass", [((1, 2, 4), False), ((2, 2, 8), False),
-                                                ((1, 2, 4), True)])
+                                                ((1, 2, 4), "batched"), ((1, 2, 4), "each")])
 def test_batch_equals_single_runs(pkg, gpu, cases, chunks, single_pass):
     """Three shapes, mixed x-chunk classes, staged DFT planes: every design's
     fields, CPML state, counters and phasors equal its own solo run bitwise,
@@ -45,7 +45,8 @@ def test_batch_equals_single_runs(pkg, gpu, cases, chunks, single_pass):
         s.run()
     batch = pkg.batch.Batch(_solvers(pkg, cases, chunks), single_pass=single_pass)
     batch.compile()
-    assert batch.use_pass == single_pass
+    assert batch.use_pass == bool(single_pass)
+    assert not single_pass or batch.pass_each == (single_pass == "each")
     batch.run()
     torch.cuda.synchronize()
     for a, b in zip(solo, batch.solvers):

@@ -11,11 +11,15 @@ device argument tables and
 captures one HIP graph per chunk of ``graph_steps`` steps that steps every
 design and replays every design's staged detectors; ``run`` replays them.
 Where every design can take the single pass, ``compile`` also times the
-batched two-step pass (``wfd_batch_pass``: two steps of every design per
-launch, each design alternating between its own two buffer sets) and uses
-it when it is faster; chunks end back in each design's first set.
+two-step pass, as one batched launch (``wfd_batch_pass``: two steps of every
+design per launch) and as one ``wfd_yee_pass`` per design in turn (each
+sized for the whole GPU), each design alternating between its own two buffer
+sets, and uses the fastest of the three; chunks end back in each design's
+first set.  ``single_pass``: "auto", True (a pass form even if slower),
+"batched" or "each" (that form), False (the fused step only).
 """
 
+import ctypes
 import time
 
 import numpy as np
@@ -48,6 +52,7 @@ class Batch:
         self._tables = {}
         self._pass_tables = {}
         self.use_pass = False
+        self.pass_each = False  # passes one design at a time instead of one batched launch
         self.pass_us = None
         self.cells = sum(s.n for s in self.solvers)
 
@@ -86,17 +91,24 @@ class Batch:
         call("wfd_batch_pass_prepare", plans, targets, n, ptr(ws), nbytes, stream())
         return plans, targets, ws
 
-    def _pass_launch(self, record, direction):
+    def _pass_launch(self, record, direction, each=None):
+        """Two steps of every design: one batched launch, or (``each``) one
+        wfd_yee_pass per design in turn, each sized for the whole GPU."""
         plans, targets, ws = self._pass_tables[(record, direction)]
-        call("wfd_batch_pass", plans, targets, len(self.solvers), ptr(ws), stream())
+        if self.pass_each if each is None else each:
+            for d in range(len(self.solvers)):
+                call("wfd_yee_pass", ctypes.byref(plans[d]), ctypes.byref(targets[d]), stream())
+        else:
+            call("wfd_batch_pass", plans, targets, len(self.solvers), ptr(ws), stream())
 
     def pass_applies(self):
         return all(s.single_pass_applies() for s in self.solvers)
 
     def tune_pass(self, steps=8):
-        """Microseconds per step of the batched two-step pass and of the
-        batched fused step (quiet tables, HIP events); keeps the pass when it
-        is faster.  Returns (pass, fused)."""
+        """Microseconds per step of the two-step pass (one batched launch, and
+        one launch per design) and of the batched fused step (quiet tables,
+        HIP events); keeps the faster pass form.  Returns (pass, fused) with
+        the faster pass form's time."""
         for s in self.solvers:
             single = s.single_pass
             s.use_single_pass(0, 1, 2, 1024)  # allocates the second buffer set and ticket
@@ -104,7 +116,8 @@ class Batch:
         self._pass_tables = {(r, d): self._pass_table(r, d) for r in (True, False) for d in (0, 1)}
         plans, ws = self._table(False)
         timings = []
-        for run in (lambda: [self._pass_launch(False, 0), self._pass_launch(False, 1)],
+        for run in (lambda: [self._pass_launch(False, 0, False), self._pass_launch(False, 1, False)],
+                    lambda: [self._pass_launch(False, 0, True), self._pass_launch(False, 1, True)],
                     lambda: [call("wfd_batch_step", plans, len(self.solvers), ptr(ws), stream())
                              for _ in range(2)]):
             run()
@@ -116,8 +129,10 @@ class Batch:
             torch.cuda.synchronize(self.device)
             timings.append(a.elapsed_time(b) * 1e3 / (2 * (steps // 2)))
         self.reset()
-        self.pass_us = {"pass": round(timings[0], 1), "fused": round(timings[1], 1)}
-        return timings[0], timings[1]
+        self.pass_each = timings[1] < timings[0]
+        self.pass_us = {"batched pass": round(timings[0], 1),
+                        "pass per design": round(timings[1], 1), "fused": round(timings[2], 1)}
+        return min(timings[0], timings[1]), timings[2]
 
     def step(self, record=True):
         """One step of every design (eager); staged detectors replay each step."""
@@ -190,7 +205,9 @@ class Batch:
         self.x_chunk_us = self.tune_x_chunk()
         if self.pass_mode is not False and self.pass_applies():
             pass_us, fused_us = self.tune_pass()
-            self.use_pass = self.pass_mode is True or pass_us < fused_us
+            self.use_pass = self.pass_mode != "auto" or pass_us < fused_us
+            if self.pass_mode in ("batched", "each"):
+                self.pass_each = self.pass_mode == "each"
         self._destroy_graphs()
         self._tables = {True: self._table(True), False: self._table(False)}
         for length in {self.graph_steps, self.steps % self.graph_steps} - {0}:
