@@ -116,18 +116,22 @@ class Batch:
         self._pass_tables = {(r, d): self._pass_table(r, d) for r in (True, False) for d in (0, 1)}
         plans, ws = self._table(False)
         timings = []
-        for run in (lambda: [self._pass_launch(False, 0, False), self._pass_launch(False, 1, False)],
-                    lambda: [self._pass_launch(False, 0, True), self._pass_launch(False, 1, True)],
-                    lambda: [call("wfd_batch_step", plans, len(self.solvers), ptr(ws), stream())
-                             for _ in range(2)]):
+        # (run, steps it advances): a pass launch advances two steps, so a
+        # pair there and back advances four.
+        for run, advance in (
+                (lambda: [self._pass_launch(False, 0, False), self._pass_launch(False, 1, False)], 4),
+                (lambda: [self._pass_launch(False, 0, True), self._pass_launch(False, 1, True)], 4),
+                (lambda: [call("wfd_batch_step", plans, len(self.solvers), ptr(ws), stream())
+                          for _ in range(2)], 2)):
             run()
             a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             a.record()
-            for _ in range(steps // 2):
+            repeats = max(1, steps // advance)
+            for _ in range(repeats):
                 run()
             b.record()
             torch.cuda.synchronize(self.device)
-            timings.append(a.elapsed_time(b) * 1e3 / (2 * (steps // 2)))
+            timings.append(a.elapsed_time(b) * 1e3 / (advance * repeats))
         self.reset()
         self.pass_each = timings[1] < timings[0]
         self.pass_us = {"batched pass": round(timings[0], 1),
