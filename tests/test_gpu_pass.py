@@ -135,3 +135,55 @@ def test_checkpointed_adjoint_on_the_single_pass_bitwise(pkg, gpu, bend, steps, 
     assert out[0][0] == out[1][0]
     np.testing.assert_array_equal(out[0][1], out[1][1])
     assert np.abs(out[1][1]).max() > 0
+
+
+@pytest.mark.parametrize("steps", [0, 1, 2])
+def test_monitors_on_one_footprint_share_a_stage(pkg, gpu, tmp_path, steps):
+    """A bend with modal and flux DFT planes on the same two planes: the four
+    staged monitors share two stage buffers, so the step stages each plane
+    once.  Their phasors equal bit for bit those of runs that stage the modal
+    planes alone and the flux planes alone (no sharing), on the fused pair
+    (steps 0) and on one- and two-step passes, and agree with per-step
+    recording (packed CPML storage, no staging; its float64 sums differ in
+    the last bits) to 1e-12."""
+    rho = np.random.default_rng(3).uniform(0, 1, (12, 12))
+    case = pkg.cases.bend_case(tmp_path / "bend", rho, shape=(48, 48, 24), spacing=40e-9,
+                               steps=150, pml=6, design=(12, 12), pulse_duration=20e-15,
+                               flux_wavelengths=[1.5e-6, 1.6e-6])
+
+    def run(keep=None):
+        s = pkg.solver.Solver(case, resident=False, single_pass=steps > 0)
+        pkg.detectors.attach(s)
+        if keep is not None:
+            s.monitors = [m for m in s.monitors if m.name.startswith(keep)]
+        s.compile()
+        if steps:
+            s.use_single_pass(0, 0, steps)
+            s.capture_all()
+        s.reset()
+        s.run()
+        torch.cuda.synchronize()
+        return s
+
+    shared = run()
+    names = [m.name for m in shared.staged()]
+    assert names == ["mode_in", "mode_out", "flux_in", "flux_out"]
+    assert shared.monitors[0].stage is shared.monitors[2].stage
+    assert shared.monitors[1].stage is shared.monitors[3].stage
+    assert shared.monitors[0].stage is not shared.monitors[1].stage
+    assert shared._plan().nstage == 2
+    alone = {m.name: m for keep in ("mode", "flux") for m in run(keep).monitors}
+    assert sorted(alone) == sorted(names)
+    for m in shared.monitors:
+        np.testing.assert_array_equal(m.phasor.numpy(), alone[m.name].phasor.numpy(),
+                                      err_msg=m.name)
+    plain = pkg.solver.Solver(case, cpml_storage="packed")
+    pkg.detectors.attach(plain)
+    plain.compile()
+    plain.reset()
+    plain.run()
+    assert plain.staged() == []
+    for a, b in zip(plain.monitors, shared.monitors):
+        assert a.name == b.name
+        assert rel_l2(b.phasor.numpy(), a.phasor.numpy()) < 1e-12, a.name
+    np.testing.assert_array_equal(plain.E.numpy(), shared.E.numpy())

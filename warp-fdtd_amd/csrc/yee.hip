@@ -174,6 +174,33 @@ __device__ __forceinline__ void stage_cell(const StageBox* st, int nstage, int s
   }
 }
 
+// The boxes (bit b for box b) that can hold cells of plane x whose flattened
+// (y, z) offsets lie in [A, B): a box spans the offsets from y0 nz + z0 up to
+// (y1 - 1) nz + z1.  Uniform in the single-pass kernels, which march one plane
+// at a time over one tile of offsets, so planes and tiles that touch no box
+// skip the per-cell test of stage_cell.
+__device__ __forceinline__ unsigned stage_hits(const StageBox* st, int nstage, int x, int A,
+                                               int B, int nz) {
+  unsigned hits = 0;
+  for (int b = 0; b < nstage; ++b) {
+    const StageBox& s = st[b];
+    if (x >= s.x0 && x < s.x1 && s.y0 * nz + s.z0 < B && (s.y1 - 1) * nz + s.z1 > A)
+      hits |= 1u << b;
+  }
+  return hits;
+}
+
+// stage_cell over the boxes of ``hits`` only.
+__device__ __forceinline__ void stage_hit_cell(const StageBox* st, unsigned hits, int slot,
+                                               int x, int y, int z, float ex, float ey, float ez,
+                                               float hx, float hy, float hz, float ox, float oy,
+                                               float oz) {
+  for (; hits; hits &= hits - 1) {
+    const int b = __builtin_ctz(hits);
+    stage_cell(st + b, 1, slot, x, y, z, ex, ey, ez, hx, hy, hz, ox, oy, oz);
+  }
+}
+
 struct Cell {
   int o, y, z, x0, nvalid;
 };
@@ -798,6 +825,7 @@ void k_yee_pass(PassArgs pa) {
       }
       // the new H of plane x - 1 over the tile
       if (h_plane) {
+        const unsigned hits = stage_hits(a.st, a.nstage, xh, A, B, nz);
 #pragma unroll
         for (int j = 0; j < J; ++j) {
           const bool own = o[j] < B;
@@ -854,9 +882,9 @@ void k_yee_pass(PassArgs pa) {
           H1.set(0, w, vx);
           H1.set(1, w, vy);
           H1.set(2, w, vz);
-          if (a.nstage && own)
-            stage_cell(a.st, a.nstage, t % a.slots, xh, y[j], z[j], ep[j][0], ep[j][1],
-                       ep[j][2], vx, vy, vz, hx, hy, hz);
+          if (hits && own)
+            stage_hit_cell(a.st, hits, t % a.slots, xh, y[j], z[j], ep[j][0], ep[j][1],
+                           ep[j][2], vx, vy, vz, hx, hy, hz);
         }
       }
 #pragma unroll
@@ -1076,6 +1104,8 @@ __device__ __forceinline__ void pass2_body(const PassArgs& pa, int bid, int nblo
         le1[x & 1][2][s] = en[j][2];
       }
       // (b) H1 of plane x - 1 over [A - nz, B + nz)
+      const unsigned hits1 =
+          xh >= x0 && xh < x1 ? stage_hits(a.st, a.nstage, xh, A, B, nz) : 0u;
 #pragma unroll
       for (int j = 0; j < J; ++j) {
         const bool on = do_b && o[j] >= 0 && o[j] < Bh;
@@ -1103,9 +1133,9 @@ __device__ __forceinline__ void pass2_body(const PassArgs& pa, int bid, int nblo
         h1n[j][0] = on ? vx : 0.f;
         h1n[j][1] = on ? vy : 0.f;
         h1n[j][2] = on ? vz : 0.f;
-        if (a.nstage && on && o[j] >= A && o[j] < B && xh >= x0 && xh < x1)
-          stage_cell(a.st, a.nstage, t % a.slots, xh, y[j], z[j], e1p[j][0], e1p[j][1],
-                     e1p[j][2], vx, vy, vz, hp0[j][0], hp0[j][1], hp0[j][2]);
+        if (hits1 && on && o[j] >= A && o[j] < B)
+          stage_hit_cell(a.st, hits1, t % a.slots, xh, y[j], z[j], e1p[j][0], e1p[j][1],
+                         e1p[j][2], vx, vy, vz, hp0[j][0], hp0[j][1], hp0[j][2]);
         lh1[0][s] = h1n[j][0];
         lh1[1][s] = h1n[j][1];
         lh1[2][s] = h1n[j][2];
@@ -1150,6 +1180,7 @@ __device__ __forceinline__ void pass2_body(const PassArgs& pa, int bid, int nblo
         le2[xh & 1][2][s] = e2n[j][2];
       }
       // (d) H2 of plane x - 2 over the tile
+      const unsigned hits2 = do_d ? stage_hits(a.st, a.nstage, xq, A, B, nz) : 0u;
 #pragma unroll
       for (int j = 0; j < J; ++j) {
         const bool on = do_d && o[j] >= A && o[j] < B;
@@ -1185,9 +1216,9 @@ __device__ __forceinline__ void pass2_body(const PassArgs& pa, int bid, int nblo
         f.h1.set(1, v, vy);
         f.h1.set(2, v, vz);
         store_psi6(f.ph1, v, r, psh1[j]);
-        if (a.nstage && on)
-          stage_cell(a.st, a.nstage, (t + 1) % a.slots, xq, y[j], z[j], e2p[j][0], e2p[j][1],
-                     e2p[j][2], vx, vy, vz, h1p[j][0], h1p[j][1], h1p[j][2]);
+        if (hits2 && on)
+          stage_hit_cell(a.st, hits2, (t + 1) % a.slots, xq, y[j], z[j], e2p[j][0], e2p[j][1],
+                         e2p[j][2], vx, vy, vz, h1p[j][0], h1p[j][1], h1p[j][2]);
       }
 #pragma unroll
       for (int j = 0; j < J; ++j) {

@@ -286,12 +286,22 @@ class Solver:
         return [m for m in self.monitors if hasattr(m, "replay_desc")][:WFD_MAX_STAGES]
 
     def _stage_buffers(self):
+        """One stage buffer per distinct stage box, {box: buffer}: monitors on
+        the same footprint (a modal and a flux plane on one plane) share it,
+        so the step stages those cells once and each replay reads them."""
+        shared = {}
         for m in self.staged():
-            box = m.stage_box()
+            box = tuple(int(v) for v in m.stage_box())
+            if box in shared:
+                m.stage = shared[box]
+                continue
             count = (box[1] - box[0]) * (box[3] - box[2]) * (box[5] - box[4])
             buf = getattr(m, "stage", None)
-            if buf is None or buf.shape != (self.graph_steps, 9, count):
+            if buf is None or buf.shape != (self.graph_steps, 9, count) or any(
+                    buf is b for b in shared.values()):
                 m.stage = Array.zeros((self.graph_steps, 9, count), device=self.device)
+            shared[box] = m.stage
+        return shared
 
     def hprev_box(self):
         """Half-open (x0, x1, y0, y1, z0, z1) of cells whose hprev is read by
@@ -339,13 +349,12 @@ class Solver:
         p.x_chunk = self.x_chunk
         p.slots = int(self.graph_steps)
         if stage:
-            self._stage_buffers()
-            staged = self.staged()
-            p.nstage = len(staged)
-            for b, m in enumerate(staged):
-                p.stages[b].buf = ptr(m.stage)
-                for a, v in enumerate(m.stage_box()):
-                    p.stages[b].box[a] = int(v)
+            shared = self._stage_buffers()
+            p.nstage = len(shared)
+            for b, (box, buf) in enumerate(shared.items()):
+                p.stages[b].buf = ptr(buf)
+                for a, v in enumerate(box):
+                    p.stages[b].box[a] = v
         return p
 
     def reset(self):
