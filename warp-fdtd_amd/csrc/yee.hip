@@ -174,17 +174,21 @@ __device__ __forceinline__ void stage_cell(const StageBox* st, int nstage, int s
   }
 }
 
-// The boxes (bit b for box b) that can hold cells of plane x whose flattened
-// (y, z) offsets lie in [A, B): a box spans the offsets from y0 nz + z0 up to
-// (y1 - 1) nz + z1.  Uniform in the single-pass kernels, which march one plane
-// at a time over one tile of offsets, so planes and tiles that touch no box
-// skip the per-cell test of stage_cell.
-__device__ __forceinline__ unsigned stage_hits(const StageBox* st, int nstage, int x, int A,
-                                               int B, int nz) {
+// Of the boxes in ``boxes`` (bit b for box b), those that can hold cells of
+// planes [xa, xb) whose flattened (y, z) offsets lie in [A, B): a box spans
+// the offsets from y0 nz + z0 up to (y1 - 1) nz + z1.  Uniform in the
+// single-pass kernels, which march one plane at a time over one tile of
+// offsets: a workgroup tests every box once against its whole chunk, then
+// each plane against the boxes that passed, so planes and tiles that touch
+// no box skip the per-cell test of stage_cell (and most workgroups read no
+// box inside the march).
+__device__ __forceinline__ unsigned stage_hits(const StageBox* st, unsigned boxes, int xa,
+                                               int xb, int A, int B, int nz) {
   unsigned hits = 0;
-  for (int b = 0; b < nstage; ++b) {
+  for (; boxes; boxes &= boxes - 1) {
+    const int b = __builtin_ctz(boxes);
     const StageBox& s = st[b];
-    if (x >= s.x0 && x < s.x1 && s.y0 * nz + s.z0 < B && (s.y1 - 1) * nz + s.z1 > A)
+    if (xa < s.x1 && xb > s.x0 && s.y0 * nz + s.z0 < B && (s.y1 - 1) * nz + s.z1 > A)
       hits |= 1u << b;
   }
   return hits;
@@ -668,6 +672,8 @@ void k_yee_pass(PassArgs pa) {
     const float wave_ey = src_on ? a.wave_e[2 * a.steps + t] : 0.f;
     const float wave_ez = src_on ? a.wave_e[a.steps + t] : 0.f;
     const bool save_t = t >= a.save_h_from;
+    // the stage boxes this tile's planes [x0, x1) can touch
+    const unsigned tile_boxes = stage_hits(a.st, (1u << a.nstage) - 1, x0, x1, A, B, nz);
     // per slot: offset, position and the values carried along x
     int o[J], y[J], z[J];
     float hp[J][3];  // H at the plane before (the step's start)
@@ -825,7 +831,7 @@ void k_yee_pass(PassArgs pa) {
       }
       // the new H of plane x - 1 over the tile
       if (h_plane) {
-        const unsigned hits = stage_hits(a.st, a.nstage, xh, A, B, nz);
+        const unsigned hits = tile_boxes ? stage_hits(a.st, tile_boxes, xh, xh + 1, A, B, nz) : 0u;
 #pragma unroll
         for (int j = 0; j < J; ++j) {
           const bool own = o[j] < B;
@@ -1017,6 +1023,8 @@ __device__ __forceinline__ void pass2_body(const PassArgs& pa, int bid, int nblo
     const float we2y = src2 ? a.wave_e[2 * a.steps + t + 1] : 0.f;
     const float we2z = src2 ? a.wave_e[a.steps + t + 1] : 0.f;
     const bool save = t + 1 >= a.save_h_from;
+    // the stage boxes this tile's planes [x0, x1) can touch
+    const unsigned tile_boxes = stage_hits(a.st, (1u << a.nstage) - 1, x0, x1, A, B, nz);
     const int xs = max(x0 - 1, 0);
     int o[J], y[J], z[J];
     float hp0[J][3], e1p[J][3], pse1[J][6], ie1[J], h1p[J][3], psh1[J][6], e2p[J][3];
@@ -1104,8 +1112,9 @@ __device__ __forceinline__ void pass2_body(const PassArgs& pa, int bid, int nblo
         le1[x & 1][2][s] = en[j][2];
       }
       // (b) H1 of plane x - 1 over [A - nz, B + nz)
-      const unsigned hits1 =
-          xh >= x0 && xh < x1 ? stage_hits(a.st, a.nstage, xh, A, B, nz) : 0u;
+      const unsigned hits1 = tile_boxes && xh >= x0 && xh < x1
+                                 ? stage_hits(a.st, tile_boxes, xh, xh + 1, A, B, nz)
+                                 : 0u;
 #pragma unroll
       for (int j = 0; j < J; ++j) {
         const bool on = do_b && o[j] >= 0 && o[j] < Bh;
@@ -1180,7 +1189,8 @@ __device__ __forceinline__ void pass2_body(const PassArgs& pa, int bid, int nblo
         le2[xh & 1][2][s] = e2n[j][2];
       }
       // (d) H2 of plane x - 2 over the tile
-      const unsigned hits2 = do_d ? stage_hits(a.st, a.nstage, xq, A, B, nz) : 0u;
+      const unsigned hits2 =
+          tile_boxes && do_d ? stage_hits(a.st, tile_boxes, xq, xq + 1, A, B, nz) : 0u;
 #pragma unroll
       for (int j = 0; j < J; ++j) {
         const bool on = do_d && o[j] >= A && o[j] < B;
