@@ -225,7 +225,8 @@ def bench_adjoint(args):
     if not (path / "manifest.json").exists():
         pkg.cases.bend_case(path, rho, shape=(200, 200, 80), steps=4000, pml=10,
                             design=(100, 100), wavelength=1.55e-6)
-    every = args.checkpoint_every or None  # 0: the whole curl history, no recompute
+    every = 64 if args.checkpoint_every is None else (args.checkpoint_every or None)
+    # (0: the whole curl history, no recompute)
     problem = pkg.design.TransmissionProblem(path, checkpoint_every=every)
     problem.value_and_grad(rho)  # compiles the graphs
     t0 = time.perf_counter()
@@ -324,8 +325,9 @@ def bench_batch(args):
     cases = pkg.cases.batch_cases(root, count=32, seed=0, steps=3000)
     gen = time.perf_counter() - t0
     log(f"bundles ready ({gen:.0f} s); compiling")
+    every = 192 if args.checkpoint_every is None else (args.checkpoint_every or None)
     bt = pkg.design.BatchTransmission([c for c, _ in cases], wavelength=1.55e-6,
-                                      checkpoint_every=192)
+                                      checkpoint_every=every)
     bt.log = log
     bt.compile()
     log("batched forward")
@@ -343,7 +345,8 @@ def bench_batch(args):
           round(fwd * 1e3 / bt.steps, 5),
           {"workload": "32 seeded bends 120x120x64..256x224x96 @ 20 nm, uniform or blurred "
                        "densities, 8-wavelength flux DFT + modal planes, 3000 steps",
-           "designs": 32, "cells": bt.cells, "forward_s": round(fwd, 3),
+           "designs": 32, "cells": bt.cells, "checkpoint_every": every,
+           "forward_s": round(fwd, 3),
            "adjoint_s": round(bwd, 3), "designs_per_s_forward_adjoint": round(32 / (fwd + bwd), 4),
            "generation_s": round(gen, 1), "T_mean": float(np.mean(T)),
            "x_chunks": sorted({s.x_chunk for s in bt.batch.solvers}),
@@ -614,9 +617,10 @@ def main():
                    help="waveguide: the headline Yee-step metric; the others measure the "
                         "adjoint, batched-design and surrogate workloads")
     p.add_argument("--fd", action="store_true", help="bend-adjoint: central-difference check")
-    p.add_argument("--checkpoint-every", type=int, default=64,
-                   help="bend-adjoint: steps between field checkpoints (0: keep the whole "
-                        "design-box curl history instead and skip the recompute)")
+    p.add_argument("--checkpoint-every", type=int, default=None,
+                   help="bend-adjoint (default 64) and batch (default 192): steps between "
+                        "field checkpoints (0: keep the whole design-box curl history "
+                        "instead and skip the recompute)")
     p.add_argument("--labels", default="",
                    help="surrogate: directories of label files (comma-separated)")
     p.add_argument("--held-out", type=int, default=64,

@@ -74,10 +74,14 @@ def test_batch_validation(pkg, gpu, cases):
         pkg.batch.Batch(s, graph_steps=32)
 
 
-def test_batched_forward_adjoint_equals_single_problems(pkg, gpu, tmp_path):
+@pytest.mark.parametrize("checkpoint,single_pass", [(128, "auto"), (None, "batched"),
+                                                   (None, False)])
+def test_batched_forward_adjoint_equals_single_problems(pkg, gpu, tmp_path, checkpoint,
+                                                        single_pass):
     """BatchTransmission (batched forwards with checkpoints between chunk
-    graphs, per-design reverses) equals each design's own
-    TransmissionProblem bit for bit: T and dT/drho."""
+    graphs, or recording every design's whole curl history on the batched
+    two-step pass or before each fused step; per-design reverses) equals
+    each design's own TransmissionProblem bit for bit: T and dT/drho."""
     rng = np.random.default_rng(7)
     cases, rhos = [], []
     for k, (shape, side) in enumerate((((48, 48, 24), 12), ((56, 48, 24), 14),
@@ -88,11 +92,16 @@ def test_batched_forward_adjoint_equals_single_problems(pkg, gpu, tmp_path):
                                          pulse_duration=20e-15,
                                          flux_wavelengths=[1.5e-6, 1.55e-6, 1.6e-6]))
         rhos.append(rng.uniform(0, 1, (side, side)))
-    bt = pkg.design.BatchTransmission(cases, wavelength=1.55e-6, checkpoint_every=128)
+    bt = pkg.design.BatchTransmission(cases, wavelength=1.55e-6, checkpoint_every=checkpoint,
+                                      single_pass=single_pass)
     T = bt.forward(rhos)
+    if single_pass == "batched":
+        assert bt.batch.use_pass and not bt.batch.pass_each
+    elif single_pass is False:
+        assert not bt.batch.use_pass
     grads = bt.backward()
     for case, rho, t, g in zip(cases, rhos, T, grads):
-        p = pkg.design.TransmissionProblem(case, wavelength=1.55e-6, checkpoint_every=128)
+        p = pkg.design.TransmissionProblem(case, wavelength=1.55e-6, checkpoint_every=checkpoint)
         t1, g1 = p.value_and_grad(rho)
         assert t == t1
         np.testing.assert_array_equal(g.cpu().numpy(), g1.cpu().numpy())

@@ -166,18 +166,27 @@ class Adjoint(Solver):
     def _state(self):
         return [self.E, self.H, self.psi_e, self.psi_h]
 
+    def history_target(self):
+        """(history, slots, design box): where a two-step pass writes the curl
+        of its steps."""
+        return (self.history, self.history.shape[0], self.box)
+
+    def record_curl(self):
+        """The curl of the current step into the history (before a fused step)."""
+        x0, y0, z0, sx, sy, sz = self.box
+        call("wfd_record_curl", ptr(self.current_h()), ptr(self.history),
+             int(self.history.shape[0]),
+             ptr(self.step), *self.shape, x0, y0, z0, sx, sy, sz, float(self.courant),
+             stream())
+
     def launch_step(self, record=True, curl=True):
         """adjoint.py:271-327: record_curl reads H before this step's H
         half-step, i.e. before the fused E+H step as well."""
         if curl and self._schedule and self._schedule[0] in ("p2", "skip"):
             # a two-step pass records the curl of both its steps itself
-            self._curl = (self.history, self.history.shape[0], self.box)
+            self._curl = self.history_target()
         elif curl:
-            x0, y0, z0, sx, sy, sz = self.box
-            call("wfd_record_curl", ptr(self.current_h()), ptr(self.history),
-                 int(self.history.shape[0]),
-                 ptr(self.step), *self.shape, x0, y0, z0, sx, sy, sz, float(self.courant),
-                 stream())
+            self.record_curl()
         self._step_fields(stage=record)
         self._curl = None
         if record:

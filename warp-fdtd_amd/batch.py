@@ -16,7 +16,10 @@ design per launch) and as one ``wfd_yee_pass`` per design in turn (each
 sized for the whole GPU), each design alternating between its own two buffer
 sets, and uses the fastest of the three; chunks end back in each design's
 first set.  ``single_pass``: "auto", True (a pass form even if slower),
-"batched" or "each" (that form), False (the fused step only).
+"batched" or "each" (that form), False (the fused step only).  Adjoint
+designs without checkpoints (``checkpoint_every`` None) record their whole
+curl history during the batched forward: the two-step pass writes it
+itself, and a fused step follows one record_curl launch per such design.
 """
 
 import ctypes
@@ -64,9 +67,17 @@ class Batch:
         call("wfd_batch_prepare", plans, len(self.solvers), ptr(ws), nbytes, stream())
         return plans, ws
 
+    def _full_history(self):
+        """The designs whose forward records the whole curl history (adjoints
+        without checkpoints): every batched step records it for them."""
+        return [s for s in self.solvers
+                if hasattr(s, "history_target") and s.checkpoint_every is None]
+
     def _pass_table(self, record, direction):
         """Plans and targets of the batched two-step pass from each design's
-        first buffer set to its second (direction 0) or back (1)."""
+        first buffer set to its second (direction 0) or back (1); the pass
+        writes the curl history of designs that keep it whole."""
+        full = set(map(id, self._full_history()))
         plans, targets = [], []
         for solver in self.solvers:
             plan = solver._plan(stage=record)
@@ -79,6 +90,11 @@ class Batch:
             out.psi_e, out.psi_h = ptr(dst["psi_e"]), ptr(dst["psi_h"])
             out.ticket = ptr(solver._ticket)
             out.steps, out.slots_per_thread, out.threads = 2, 1, 1024
+            if id(solver) in full:
+                hist, slots, box = solver.history_target()
+                out.hist, out.hist_steps = ptr(hist), int(slots)
+                for q, v in enumerate(box):
+                    out.hist_box[q] = int(v)
             plans.append(plan)
             targets.append(out)
         n = len(self.solvers)
@@ -101,6 +117,13 @@ class Batch:
         else:
             call("wfd_batch_pass", plans, targets, len(self.solvers), ptr(ws), stream())
 
+    def _fused_step(self, plans, ws, full):
+        """One batched fused step; the designs in ``full`` record their curl
+        first, as their own forward does."""
+        for solver in full:
+            solver.record_curl()
+        call("wfd_batch_step", plans, len(self.solvers), ptr(ws), stream())
+
     def pass_applies(self):
         return all(s.single_pass_applies() for s in self.solvers)
 
@@ -115,14 +138,14 @@ class Batch:
             s.single_pass = single
         self._pass_tables = {(r, d): self._pass_table(r, d) for r in (True, False) for d in (0, 1)}
         plans, ws = self._table(False)
+        full = self._full_history()
         timings = []
         # (run, steps it advances): a pass launch advances two steps, so a
         # pair there and back advances four.
         for run, advance in (
                 (lambda: [self._pass_launch(False, 0, False), self._pass_launch(False, 1, False)], 4),
                 (lambda: [self._pass_launch(False, 0, True), self._pass_launch(False, 1, True)], 4),
-                (lambda: [call("wfd_batch_step", plans, len(self.solvers), ptr(ws), stream())
-                          for _ in range(2)], 2)):
+                (lambda: [self._fused_step(plans, ws, full) for _ in range(2)], 2)):
             run()
             a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             a.record()
@@ -141,13 +164,14 @@ class Batch:
     def step(self, record=True):
         """One step of every design (eager); staged detectors replay each step."""
         plans, ws = self._tables[record]
-        call("wfd_batch_step", plans, len(self.solvers), ptr(ws), stream())
+        self._fused_step(plans, ws, self._full_history())
         if record:
             for s in self.solvers:
                 s._replay_staged(1)
 
     def _capture(self, length, record):
         plans, ws = self._tables[record]
+        full = self._full_history()
         torch.cuda.synchronize(self.device)
         s = self._handles.begin()
         ext = torch.cuda.ExternalStream(s.value, device=self.device)
@@ -162,7 +186,7 @@ class Batch:
                     for k in range(passes):
                         self._pass_launch(record, k % 2)
                     for _ in range(m - 2 * passes):
-                        call("wfd_batch_step", plans, len(self.solvers), ptr(ws), stream())
+                        self._fused_step(plans, ws, full)
                     if record:
                         for solver in self.solvers:
                             solver._replay_staged(m)

@@ -169,21 +169,25 @@ class BatchTransmission:
     steps all grids, and each design's field checkpoint is saved between
     chunks every ``checkpoint_every`` steps (a multiple of ``graph_steps``).
     Each design's reverse then runs on its own recompute/reverse graphs.
-    Values and gradients are bitwise those of the designs' own
-    ``TransmissionProblem``s.
+    With ``checkpoint_every`` None the batched forward records every design's
+    whole design-box curl history instead (steps x 3 x box cells floats per
+    design) and the reverses recompute nothing.  ``single_pass`` is the
+    batch's (``Batch``).  Values and gradients are bitwise those of the
+    designs' own ``TransmissionProblem``s.
     """
 
     def __init__(self, cases, wavelength=None, checkpoint_every=192, graph_steps=64,
-                 device="cuda:0"):
+                 device="cuda:0", single_pass="auto"):
         from .batch import Batch
 
-        if checkpoint_every is None or checkpoint_every % graph_steps:
-            raise ValueError("checkpoint_every must be a multiple of graph_steps")
+        if checkpoint_every is not None and checkpoint_every % graph_steps:
+            raise ValueError("checkpoint_every must be a multiple of graph_steps (or None)")
         self.problems = [TransmissionProblem(c, wavelength=wavelength,
                                              checkpoint_every=checkpoint_every, device=device,
                                              graph_steps=graph_steps, resident=False)
                          for c in cases]
-        self.batch = Batch([p.adjoint for p in self.problems], graph_steps=graph_steps)
+        self.batch = Batch([p.adjoint for p in self.problems], graph_steps=graph_steps,
+                           single_pass=single_pass)
         self.checkpoint_every = checkpoint_every
         self.graph_steps = graph_steps
         self.cells = self.batch.cells
@@ -212,12 +216,13 @@ class BatchTransmission:
         active = np.zeros(self.steps, bool)
         for p in self.problems:
             active |= p.adjoint._windows()
-        K, G = self.checkpoint_every, self.graph_steps
+        K, G = self.checkpoint_every or self.steps, self.graph_steps
         s = stream()
         for c, start in enumerate(range(0, self.steps, K)):
-            for p in self.problems:
-                for dst, src in zip(p.adjoint.checkpoints[c], p.adjoint._state()):
-                    dst.tensor.copy_(src.tensor)
+            if self.checkpoint_every is not None:
+                for p in self.problems:
+                    for dst, src in zip(p.adjoint.checkpoints[c], p.adjoint._state()):
+                        dst.tensor.copy_(src.tensor)
             t, end = start, min(start + K, self.steps)
             while t < end:
                 length = min(G, end - t)
