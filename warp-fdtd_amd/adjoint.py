@@ -451,6 +451,12 @@ class Adjoint(Solver):
         return {self.checkpoint_every, self.steps % self.checkpoint_every} - {0}
 
     def backward(self):
+        self.launch_backward()
+        torch.cuda.synchronize(self.device)
+
+    def launch_backward(self):
+        """Enqueue the whole reverse on the current stream (no wait): with
+        checkpoints, each segment's state restore, recompute and reverse."""
         s = stream()
         if self._turns():
             self.inject(0)  # the first step's; every turn applies the next one's
@@ -459,7 +465,6 @@ class Adjoint(Solver):
                 call("wfd_graph_launch", self.reverse_graphs[self.graph_steps], s)
             if self.steps % self.graph_steps:
                 call("wfd_graph_launch", self.reverse_graphs[self.steps % self.graph_steps], s)
-            torch.cuda.synchronize(self.device)
             return
         K = self.checkpoint_every
         starts = list(range(0, self.steps, K))
@@ -471,7 +476,6 @@ class Adjoint(Solver):
             self.counters.tensor[0] = t0
             call("wfd_graph_launch", self._recompute_graphs[length], s)
             call("wfd_graph_launch", self.reverse_graphs[length], s)
-        torch.cuda.synchronize(self.device)
 
     def set_design(self, parameters):
         from .material import Material

@@ -110,19 +110,29 @@ class TransmissionProblem:
         r = self.map._rho(rho, self.adjoint.device)
         if self._forward_rho is None or not torch.equal(r, self._forward_rho):
             self.forward(r)
-        a = self.adjoint
         t0 = time.perf_counter()
+        self._prepare_backward()
+        self.adjoint.backward()
+        out = self._finish_backward(r)
+        self.timings["backward_s"] = time.perf_counter() - t0
+        return out
+
+    def _prepare_backward(self):
+        """The reverse's setup after a forward: adjoint sources, reverse
+        graphs, zeroed reverse state."""
+        a = self.adjoint
         a.prepare_sources()
         if not a.reverse_graphs:
             # the reverse graphs capture the cotangent buffers, which exist
             # once prepare_sources has run; later calls refresh them in place
             a.compile_reverse()
         a.reset_reverse()
-        a.backward()
-        # the adjoint accumulates d(-T)/d(inv_eps)
-        out = -self.map.pullback(r, a.grad)
-        torch.cuda.synchronize(a.device)
-        self.timings["backward_s"] = time.perf_counter() - t0
+
+    def _finish_backward(self, r):
+        """dT/drho from the finished reverse (the adjoint accumulates
+        d(-T)/d(inv_eps))."""
+        out = -self.map.pullback(r, self.adjoint.grad)
+        torch.cuda.synchronize(self.adjoint.device)
         return out
 
     def value_and_grad(self, rho):
@@ -236,13 +246,24 @@ class BatchTransmission:
             p._forward_rho = r.clone()
         return [p.adjoint.transmission() for p in self.problems]
 
-    def backward(self):
-        """dT/drho of every design for the densities of the last forward."""
+    def backward(self, streams=2):
+        """dT/drho of every design for the densities of the last forward.
+        Every design's reverse is set up first; the reverses then run on
+        ``streams`` streams in turn, so that one design's latency-bound
+        launches (the inject turn) overlap another's reverse halves (the
+        designs share no buffers)."""
         t0 = time.perf_counter()
-        grads = []
         for k, p in enumerate(self.problems):
-            grads.append(p.backward(p._forward_rho))
+            p._prepare_backward()
             if k % 8 == 7:
-                self.log(f"adjoint {k + 1}/{len(self.problems)}")
+                self.log(f"adjoint set up {k + 1}/{len(self.problems)}")
+        lanes = [torch.cuda.Stream(device=self.problems[0].adjoint.device)
+                 for _ in range(max(1, int(streams)))]
+        torch.cuda.synchronize(self.problems[0].adjoint.device)
+        for k, p in enumerate(self.problems):
+            with torch.cuda.stream(lanes[k % len(lanes)]):
+                p.adjoint.launch_backward()
+        torch.cuda.synchronize(self.problems[0].adjoint.device)
+        grads = [p._finish_backward(p._forward_rho) for p in self.problems]
         self.timings["backward_s"] = time.perf_counter() - t0
         return grads
