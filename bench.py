@@ -691,7 +691,8 @@ def main():
         per_launch_steps = int(np.median(lengths))
         mean_s = float(seconds.sum() / lengths.sum())
         median_s = float(np.median(seconds / lengths))
-        kernel = f"k_resident ({len(launches)} launches of {sorted(set(lengths.astype(int)))} steps)"
+        counts = sorted({int(v) for v in lengths})
+        kernel = f"k_resident ({len(launches)} launches of {counts} steps)"
         timing = ("HIP events on the launch stream around every resident launch of the timed "
                   "runs (each with its counter kernel)")
     else:
