@@ -121,6 +121,23 @@ def test_nonfinite_counter(pkg, gpu):
     assert int(count.item()) == 2
 
 
+def test_assert_finite_on_a_stepped_guide(pkg, gpu):
+    """guide_validation.py:12-20: passes on a stepped DrivenUniform; one
+    non-finite value in e or h raises FloatingPointError naming the count."""
+    from warp_fdtd_amd.driven_uniform import DrivenUniform
+    from warp_fdtd_amd.guide_problem import GuideProblem
+    from warp_fdtd_amd.guide_validation import assert_finite
+
+    s = DrivenUniform(GuideProblem(), 0.2, 16, layout="flat")
+    s.compile()
+    s.run(8)
+    assert_finite(s)
+    s.h.tensor[3] = float("nan")
+    s.e.tensor[5] = float("inf")
+    with pytest.raises(FloatingPointError, match="2 nonfinite"):
+        assert_finite(s)
+
+
 def test_periodic_dispersion_and_gauss_law(pkg, gpu):
     """test_periodic.py:10-46 (Yee branch): oblique periodic mode frequency
     equals 2/dt asin(S dt/2) within 2e-6 and the discrete divergence stays
