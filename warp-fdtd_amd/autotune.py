@@ -195,10 +195,10 @@ def _pass_fits(nz, steps, slots, threads):
     return (nz if steps == 1 else 3 * nz) < slots * threads
 
 
-def measure_pass(solver, planes, slots, steps=1, threads=0, repeats=20):
-    """Median microseconds per step of wfd_yee_pass launches (a pair there
-    and back) with the given shape; runs real steps (reset after)."""
-    solver.use_single_pass(planes, slots, steps, threads)
+def measure_pass(solver, planes, slots, steps=1, threads=0, repeats=20, table=True):
+    """Median microseconds per step of single-pass launches (a pair there
+    and back) with the given shape and form; runs real steps (reset after)."""
+    solver.use_single_pass(planes, slots, steps, threads, table)
     kinds = ["p2", "skip", "p2", "skip"] if steps == 2 else ["p1", "p1"]
     try:
         def pair():
@@ -222,10 +222,13 @@ def measure_pass(solver, planes, slots, steps=1, threads=0, repeats=20):
 
 
 def tune_pass(solver, use_cache=True, require=False):
-    """(planes, slots per thread, steps per launch) of the fastest single-pass
-    shape if it beats the tuned fused step on this grid (or in any case with
-    ``require``), else None."""
-    key = _key(solver) + "|single-pass-6"
+    """(planes, slots per thread, steps per launch, threads, table) of the
+    fastest single-pass shape and form if it beats the tuned fused step on
+    this grid (or in any case with ``require``), else None.  The shapes are
+    timed in the table form where it applies (two-step passes of one slot per
+    1024-thread workgroup, launched as a batch of one); the best is then timed
+    with its arguments in the kernarg segment too, and the faster form kept."""
+    key = _key(solver) + "|single-pass-7"
     if use_cache:
         hit = _memory.get(key)
         if hit is None:
@@ -243,7 +246,15 @@ def tune_pass(solver, use_cache=True, require=False):
     fused = measure(solver, solver.x_chunk or 2)
     timings = {"x".join(map(str, sh)): measure_pass(solver, *sh) for sh in shapes}
     best = min(shapes, key=lambda sh: timings["x".join(map(str, sh))])
-    use = timings["x".join(map(str, best))] < fused
+    best_us = timings["x".join(map(str, best))]
+    table = best[2] == 2 and best[1] in (0, 1) and best[3] in (0, 1024)
+    if table:
+        kernarg = measure_pass(solver, *best, table=False)
+        timings["x".join(map(str, best)) + " kernarg"] = kernarg
+        table = best_us <= kernarg
+        best_us = min(best_us, kernarg)
+    best = (*best, table)
+    use = best_us < fused
     entry = {"single_pass": use, "shape": list(best),
              "us": {"fused": round(fused, 3), **{k: round(v, 3) for k, v in timings.items()}}}
     _memory[key] = entry

@@ -270,6 +270,8 @@ class Solver:
         self._phase = 0           # 1 while the state is in the second set (inside a capture)
         self._schedule = []       # launch kind of each step left in the capture
         self._curl = None         # (history, slots, box) a two-step pass records
+        self._pass_tables = {}    # device argument tables of two-step passes (batch of one)
+        self.pass_table = False
         self._compiled = False
 
     # ------------------------------------------------------------ stepping
@@ -415,10 +417,12 @@ class Solver:
         return (self.cpml_storage == "full" and len(self.staged()) == len(self.monitors)
                 and self.shape[2] < 1024)
 
-    def use_single_pass(self, planes=0, slots_per_thread=0, steps=1, threads=0):
+    def use_single_pass(self, planes=0, slots_per_thread=0, steps=1, threads=0, table=True):
         """Step captured chunks with wfd_yee_pass, ``steps`` (1 or 2) steps per
         launch, ``threads`` per workgroup (0: the library's choice for the
-        shape; allocates the second buffer set)."""
+        shape; allocates the second buffer set).  ``table``: two-step passes
+        of one slot per 1024-thread workgroup launch as a batch of one
+        (wfd_batch_pass, arguments in a device table)."""
         if not self.single_pass_applies():
             raise ValueError("the single-pass step needs full CPML storage, staged monitors "
                              "only and nz < 1024")
@@ -431,7 +435,9 @@ class Solver:
                          "psi_h": Array.zeros(self.psi_h.shape, device=self.device)}
             self._ticket = Array.zeros(1, np.int32, self.device)
         self.pass_shape = (int(planes), int(slots_per_thread), int(steps), int(threads))
+        self.pass_table = bool(table)
         self.single_pass = True
+        self._prepare_pass_tables()
 
     def _pass_schedule(self, length, two_step=True):
         """Launch kind of each step of a captured chunk of ``length`` steps:
@@ -451,10 +457,6 @@ class Solver:
             left -= 2
         return kinds + ["fused"] * left
 
-    def _sets(self):
-        main = {"E": self.E, "H": self.H, "psi_e": self.psi_e, "psi_h": self.psi_h}
-        return (main, self._alt) if self._phase == 0 else (self._alt, main)
-
     def _pass_step(self, stage):
         """The next step of the capture's schedule: a wfd_yee_pass from the
         current set to the other (one or two steps), nothing (the second step
@@ -467,7 +469,19 @@ class Solver:
                 raise RuntimeError("a fused step with the state in the second set")
             k.yee_step(self._plan(stage))
             return
-        src, dst = self._sets()
+        plan, out = self._pass_args(stage, self._phase, 2 if kind == "p2" else 1, self._curl)
+        table = self._pass_tables.get(bytes(plan) + bytes(out))
+        if table is not None:
+            call("wfd_batch_pass", ctypes.byref(table[0]), ctypes.byref(table[1]), 1,
+                 ptr(table[2]), stream())
+        else:
+            call("wfd_yee_pass", ctypes.byref(plan), ctypes.byref(out), stream())
+        self._phase ^= 1
+
+    def _pass_args(self, stage, phase, steps, curl):
+        """(plan, targets) of a pass from buffer set ``phase`` to the other."""
+        main = {"E": self.E, "H": self.H, "psi_e": self.psi_e, "psi_h": self.psi_h}
+        src, dst = (main, self._alt) if phase == 0 else (self._alt, main)
         plan = self._plan(stage)
         plan.e, plan.h = ptr(src["E"]), ptr(src["H"])
         plan.psi_e, plan.psi_h = ptr(src["psi_e"]), ptr(src["psi_h"])
@@ -475,17 +489,40 @@ class Solver:
         out.e, out.h = ptr(dst["E"]), ptr(dst["H"])
         out.psi_e, out.psi_h = ptr(dst["psi_e"]), ptr(dst["psi_h"])
         out.ticket = ptr(self._ticket)
-        out.steps = 2 if kind == "p2" else 1
+        out.steps = steps
         if out.steps == self.pass_shape[2]:
             out.planes, out.slots_per_thread = self.pass_shape[:2]
             out.threads = self.pass_shape[3]
-        if kind == "p2" and self._curl is not None:
-            hist, slots, box = self._curl
+        if steps == 2 and curl is not None:
+            hist, slots, box = curl
             out.hist, out.hist_steps = ptr(hist), int(slots)
             for q, v in enumerate(box):
                 out.hist_box[q] = int(v)
-        call("wfd_yee_pass", ctypes.byref(plan), ctypes.byref(out), stream())
-        self._phase ^= 1
+        return plan, out
+
+    def _prepare_pass_tables(self):
+        """Two-step passes of one offset per 1024-thread slot launch as a batch
+        of one (wfd_batch_pass): the kernel reads its arguments from a device
+        table instead of the kernarg segment and keeps them out of the scalar
+        registers it would otherwise spill.  The tables (per stage flag,
+        buffer set and curl target) are written here, outside any capture;
+        a pass whose arguments match none takes wfd_yee_pass."""
+        self._pass_tables = {}
+        planes, slots, steps, threads = self.pass_shape
+        if not self.pass_table or steps != 2 or slots not in (0, 1) or threads not in (0, 1024):
+            return
+        from ._lib import lib
+
+        nbytes = int(lib().wfd_batch_pass_workspace_bytes(1))
+        curls = [None] + ([self.history_target()] if hasattr(self, "history_target") else [])
+        for stage in (True, False):
+            for phase in (0, 1):
+                for curl in curls:
+                    plan, out = self._pass_args(stage, phase, 2, curl)
+                    ws = Array.zeros((nbytes + 3) // 4, np.int32, self.device)
+                    call("wfd_batch_pass_prepare", ctypes.byref(plan), ctypes.byref(out), 1,
+                         ptr(ws), nbytes, stream())
+                    self._pass_tables[bytes(plan) + bytes(out)] = (plan, out, ws)
 
     def current_h(self):
         """H of the state the next step starts from (the second set while a
