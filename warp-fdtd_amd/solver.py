@@ -458,8 +458,9 @@ class Solver:
         return kinds + ["fused"] * left
 
     def _pass_step(self, stage):
-        """The next step of the capture's schedule: a wfd_yee_pass from the
-        current set to the other (one or two steps), nothing (the second step
+        """The next step of the capture's schedule: a pass from the current
+        set to the other (one or two steps; wfd_yee_pass, or wfd_batch_pass
+        with the prepared table of these arguments), nothing (the second step
         of a two-step launch) or the fused pair in place."""
         kind = self._schedule.pop(0)
         if kind == "skip":
@@ -501,12 +502,13 @@ class Solver:
         return plan, out
 
     def _prepare_pass_tables(self):
-        """Two-step passes of one offset per 1024-thread slot launch as a batch
-        of one (wfd_batch_pass): the kernel reads its arguments from a device
-        table instead of the kernarg segment and keeps them out of the scalar
-        registers it would otherwise spill.  The tables (per stage flag,
-        buffer set and curl target) are written here, outside any capture;
-        a pass whose arguments match none takes wfd_yee_pass."""
+        """With ``pass_table``, two-step passes of one slot per thread in
+        1024-thread workgroups launch as a batch of one (wfd_batch_pass): the
+        kernel reads its arguments from a device table instead of the kernarg
+        segment and keeps them out of the scalar registers it would otherwise
+        spill.  The tables (per stage flag, buffer set and curl target) are
+        written here, outside any capture; a pass whose arguments match none
+        takes wfd_yee_pass."""
         self._pass_tables = {}
         planes, slots, steps, threads = self.pass_shape
         if not self.pass_table or steps != 2 or slots not in (0, 1) or threads not in (0, 1024):
